@@ -1,0 +1,189 @@
+/*
+ * mh.h — C ABI of the MI355X ray-traversal library (libmh.so) behind Manta's
+ * DynBVH plugin interface.
+ *
+ * Every entry point returns MH_OK (0) or a negative MH_E* code; on failure
+ * mh_last_error() returns a thread-local message. No C++ exception crosses this
+ * boundary. All array arguments are plain pointers with explicit sizes; device
+ * pointers are HIP device addresses on the device chosen by mh_init().
+ *
+ * Reference interfaces replaced (paths are in the rfManta tree):
+ *   AccelerationStructure  Interface/AccelerationStructure.h:11-66
+ *   Object::intersect      Interface/Object.h:34
+ *   DynBVH                 Model/Groups/DynBVH.h:22-391, DynBVH.cc:101-2501
+ *   RayPacket hit/mask     Interface/RayPacket.h:407-442,488-502,531-657
+ */
+#ifndef MH_H_
+#define MH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_ABI_VERSION 1
+
+/* Status codes. */
+#define MH_OK 0
+#define MH_E_INVALID (-1)     /* bad argument (null pointer, size, flag) */
+#define MH_E_HIP (-2)         /* a HIP runtime call failed */
+#define MH_E_NOMEM (-3)       /* device or host allocation failed */
+#define MH_E_STATE (-4)       /* call out of order (e.g. trace before build) */
+#define MH_E_UNSUPPORTED (-5) /* input outside a documented limit */
+#define MH_E_IO (-6)          /* file could not be read or written */
+
+/* Trace flags. */
+#define MH_CLOSEST_HIT 0u
+#define MH_ANY_HIT 1u /* RayPacket::AnyHit (Interface/RayPacket.h:148); opaque occluders */
+
+/* Triangle tests (Model/Primitives). */
+#define MH_PRIM_KENSLER_SHIRLEY 0 /* KenslerShirleyTriangle.cc:134-377 */
+#define MH_PRIM_WALD 1            /* WaldTriangle.cc:64-555 */
+
+/* Sentinel values of mh_hits.prim. */
+#define MH_NO_HIT (-1)
+#define MH_MASKED (-2)
+
+/* Reference constants (include/CMakeLists.txt:185,188). */
+#define MH_T_EPSILON 1.e-4f
+#define MH_MAXT 1.e19f
+
+typedef struct mh_scene mh_scene;
+
+/* Rays in structure-of-arrays form, one float per ray per array, exactly the
+ * RayPacketData origin/direction/minT fields (Interface/RayPacket.h:92-97).
+ * tmax is the ray's initial minT (MH_MAXT for camera rays, 1 - MH_T_EPSILON for
+ * shadow rays, Model/Lights/PointLight.cc:42). masked may be NULL; a nonzero byte
+ * marks a ray as masked (RayPacket::maskRay, RayPacket.h:488-495): it never hits
+ * and reports prim == MH_MASKED and, for MH_ANY_HIT, occluded == 1. */
+typedef struct mh_rays {
+  const float* ox;
+  const float* oy;
+  const float* oz;
+  const float* dx;
+  const float* dy;
+  const float* dz;
+  const float* tmax;
+  const uint8_t* masked;
+} mh_rays;
+
+/* Hit records in structure-of-arrays form. For MH_CLOSEST_HIT: prim (triangle
+ * index in the mesh, or MH_NO_HIT / MH_MASKED), t (the final minT), u and v
+ * (barycentric weights of vertex 1 and vertex 2, the SCRATCH_U/SCRATCH_V
+ * scratchpad slots). For MH_ANY_HIT: occluded (RayPacket::wasHit) and, if
+ * non-NULL, prim (the occluder found first). Any pointer except the one the
+ * mode needs may be NULL. */
+typedef struct mh_hits {
+  int32_t* prim;
+  float* t;
+  float* u;
+  float* v;
+  uint8_t* occluded;
+} mh_hits;
+
+/* Trace options; NULL means the defaults given in brackets. */
+typedef struct mh_trace_opts {
+  /* [1] 1 = each ray traverses alone (DynBVH::intersect on a one-ray packet);
+   * 2..64 = consecutive groups of packet_size rays traverse as one RayPacket on
+   * one 64-lane wave (DynBVH::intersectNode packet semantics). */
+  int packet_size;
+  /* [0] per-lane traversal stack entries held in LDS (8,16,32; 0 = autotuned). */
+  int stack_lds;
+  /* [0] persistent blocks per CU (0 = autotuned). */
+  int blocks_per_cu;
+  /* [NULL] optional device int64[2]: box tests, triangle tests (accumulated). */
+  int64_t* counters;
+} mh_trace_opts;
+
+/* Build options. */
+typedef struct mh_build_opts {
+  int prim_kind; /* MH_PRIM_* [MH_PRIM_KENSLER_SHIRLEY] */
+  int rtsah;     /* 1 = compute isLeftCheaper (DynBVH.cc:1583-1678) [1] */
+  int threads;   /* host threads for the SAH build, 0 = all cores */
+} mh_build_opts;
+
+/* Library. */
+int mh_init(int device);                 /* select the HIP device; idempotent */
+const char* mh_last_error(void);          /* thread-local message of the last failure */
+int mh_abi_version(void);
+int mh_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes);
+
+/* Scene life cycle — AccelerationStructure::setGroup (AccelerationStructure.h:20)
+ * on a Mesh (Model/Groups/Mesh.h:17-118): copies the host vertex (3 floats each)
+ * and index (3 uint32 each) arrays. */
+int mh_scene_create(const float* verts, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris,
+                    mh_scene** out);
+int mh_scene_destroy(mh_scene* scene);
+
+/* DynBVH::rebuild(proc, numProcs) with group_changed (DynBVH.cc:1699-1793): full SAH
+ * build on the host (same split choices as partitionSAH/buildEvents, DynBVH.cc:2310-2440),
+ * optional RTSAH, then upload. */
+int mh_scene_build(mh_scene* scene, const mh_build_opts* opts);
+
+/* The same host build without a scene or a device: DynBVH::rebuild (DynBVH.cc:1699-1793)
+ * writing object_ids (n_tris ints) and 2*n_tris 32-byte BVHNode records (DynBVH.h:37-77,
+ * the array size DynBVH::allocate uses, DynBVH.cc:1680-1697). *num_nodes receives the
+ * number of records in use. Touches no GPU. */
+int mh_dynbvh_build_host(const float* verts, uint32_t n_verts, const uint32_t* idx,
+                         uint32_t n_tris, const mh_build_opts* opts, int32_t* ids, void* nodes,
+                         uint32_t* num_nodes);
+
+/* DynBVH::buildFromFile (DynBVH.cc:2442-2472) from memory: n_ids object ids and n_nodes
+ * 32-byte BVHNode records in the reference layout (DynBVH.h:37-77). */
+int mh_scene_load_dynbvh(mh_scene* scene, const int32_t* ids, uint32_t n_ids, const void* nodes,
+                         uint32_t n_nodes, int prim_kind);
+/* DynBVH::buildFromFile / saveToFile (DynBVH.cc:2442-2501) on a file path. */
+int mh_scene_load_dynbvh_file(mh_scene* scene, const char* path, int prim_kind);
+int mh_scene_save_dynbvh_file(const mh_scene* scene, const char* path);
+/* Copies the current tree out in the reference layout (sizes first when arrays are NULL). */
+int mh_scene_export_dynbvh(const mh_scene* scene, int32_t* ids, uint32_t* n_ids, void* nodes,
+                           uint32_t* n_nodes);
+
+/* Scene queries: bounds (Object::computeBounds, Object.h:33) and tree statistics. */
+int mh_scene_bounds(const mh_scene* scene, float lo[3], float hi[3]);
+int mh_scene_stats(const mh_scene* scene, uint32_t* n_nodes, uint32_t* n_inner, uint32_t* depth,
+                   size_t* device_bytes);
+
+/* Per-frame update — AccelerationStructure::update/rebuild without group change
+ * (DynBVH.cc:1574-1581, 2012-2123): replace the vertex positions (same topology),
+ * then refit the tree bottom-up on the GPU. set_vertices copies from host memory
+ * (on_device = 0) or from device memory (on_device = 1) on `stream`. */
+int mh_scene_set_vertices(mh_scene* scene, const float* verts, int on_device, void* stream);
+int mh_scene_refit(mh_scene* scene, void* stream);
+/* Device pointer to the scene's vertex array (3 floats per vertex), for callers that
+ * animate in place; call mh_scene_refit afterwards. */
+int mh_scene_vertices_device(mh_scene* scene, float** verts);
+/* Sine displacement used by the dynamic-scene workload: y = y0 + amp*sin(freq*x + phase)
+ * from the base positions given at create time, then no refit. */
+int mh_scene_displace_sine(mh_scene* scene, float amp, float freq, float phase, void* stream);
+
+/* DynBVH::intersect (DynBVH.cc:101-404) on n rays resident in device memory.
+ * Asynchronous on `stream` (NULL = the null stream). */
+int mh_trace(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
+             uint32_t flags, const mh_trace_opts* opts, void* stream);
+
+/* Same, from and to host memory; stages through the scene's device buffers and
+ * returns when the results are on the host. */
+int mh_trace_host(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
+                  uint32_t flags, const mh_trace_opts* opts);
+
+/* Device memory helpers for callers without their own allocator. */
+int mh_malloc(void** ptr, size_t bytes);
+int mh_free(void* ptr);
+int mh_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int mh_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int mh_stream_create(void** stream);
+int mh_stream_destroy(void* stream);
+int mh_stream_sync(void* stream);
+/* Elapsed milliseconds of `iters` back-to-back mh_trace calls measured with HIP events
+ * recorded on `stream` (the stream the kernels run on). */
+int mh_time_trace(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
+                  uint32_t flags, const mh_trace_opts* opts, void* stream, int iters, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MH_H_ */

@@ -1,0 +1,1114 @@
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement of Manta's DynBVH hot path, used by the parity tests, by
+// __graft_entry__.smoke() and by the cpu_baseline leg of bench.py as the checker.
+// The product library (rfmanta_amd/libmh.so) never links, loads or calls it.
+//
+// What it restates (reference file:line):
+//   * SAH build: DynBVH::rebuild/build/partitionSAH/buildEvents/partition2Objs
+//     (Model/Groups/DynBVH.cc:1699-1857, 2277-2440), BBox (Core/Geometry/BBox.h:45-130),
+//     Mesh::getBBox (Model/Groups/Mesh.cc:228-237), Mesh::computeBounds (Mesh.cc:195-225).
+//   * RTSAH child order for any-hit: DynBVH::computeSubTreeTraversalCost
+//     (DynBVH.cc:1594-1678) with TreeTraversalProb::computeProbs
+//     (Model/Groups/TreeTraversalProbability.cc:3-95, .h:13-143), fast_atan2f
+//     (Core/Math/Trig.h:153-178) and ICSI_log (Core/Math/ExponSSE.h:20-73).
+//   * Packet traversal: DynBVH::intersect/intersectNode/firstIntersects/lastIntersects,
+//     the scalar (non-MANTA_SSE) branches (DynBVH.cc:101-404, 655-725, 882-905),
+//     RayPacket::hit/maskRay/wasHit (Interface/RayPacket.h:407-442,488-502,531-542),
+//     RayPacket::actualComputeInverseDirections scalar branch (Interface/RayPacket.cc:105-109).
+//   * Kensler-Shirley test, scalar branch (Model/Primitives/KenslerShirleyTriangle.cc:336-377).
+//   * Wald test: setPoints (Model/Primitives/WaldTriangle.cc:64-127), scalar intersect
+//     (WaldTriangle.cc:489-555).
+//   * Refit without tree rotations: DynBVH::update -> parallelUpdateBounds(numProcs==1)
+//     -> updateBounds<false> (DynBVH.cc:1574-1581, 2012-2123).
+//   * DynBVH binary file layout (DynBVH.cc:2442-2501): u32 n_ids, i32 ids[], u32 n_nodes,
+//     32-byte nodes.
+//
+// Numerics: every float operation is IEEE binary32 with round-to-nearest and no
+// contraction (g++ -O2 on baseline x86-64 emits no FMA). The scalar build of the
+// reference is followed; the SSE build's rcpps-based reciprocals are CPU-model specific
+// and are not restated (see the parity hazard list in the design notes).
+//
+// Parity pins: tests/test_oracle_pins.py checks this file against brute force over all
+// triangles, double-precision closed forms, hand-computed known-answer rays and the BVH
+// invariants; see that file for which function each check pins.
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// include/CMakeLists.txt:185,188 with include/Parameters.h.CMakeTemplate:33,36.
+const float kTEps = (float)1.e-4;
+const float kMaxT = (float)1.e19;
+// DynBVH.cc:29-30
+const float kCIsec = 1.f;
+const float kCTrav = 1.f;
+
+struct Vec {
+  float v[3];
+};
+inline Vec vsub(const Vec& a, const Vec& b) {
+  Vec r;
+  r.v[0] = a.v[0] - b.v[0];
+  r.v[1] = a.v[1] - b.v[1];
+  r.v[2] = a.v[2] - b.v[2];
+  return r;
+}
+// Core/Geometry/Vector.h:526-531 (left-to-right sum)
+inline float vdot(const Vec& a, const Vec& b) {
+  return a.v[0] * b.v[0] + a.v[1] * b.v[1] + a.v[2] * b.v[2];
+}
+// Core/Geometry/Vector.h:534-539
+inline Vec vcross(const Vec& a, const Vec& b) {
+  Vec r;
+  r.v[0] = a.v[1] * b.v[2] - a.v[2] * b.v[1];
+  r.v[1] = a.v[2] * b.v[0] - a.v[0] * b.v[2];
+  r.v[2] = a.v[0] * b.v[1] - a.v[1] * b.v[0];
+  return r;
+}
+// Core/Math/MinMax.h:83-91
+inline float fmin_(float a, float b) { return a < b ? a : b; }
+inline float fmax_(float a, float b) { return a > b ? a : b; }
+
+// Core/Geometry/BBox.h
+struct Box {
+  float lo[3], hi[3];
+  void reset() {
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = FLT_MAX;
+      hi[i] = -FLT_MAX;
+    }
+  }
+  void extendPoint(const float* p) {
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = fmin_(lo[i], p[i]);
+      hi[i] = fmax_(hi[i], p[i]);
+    }
+  }
+  void extendBox(const Box& b) {
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = fmin_(lo[i], b.lo[i]);
+      hi[i] = fmax_(hi[i], b.hi[i]);
+    }
+  }
+  // BBox::computeArea returns double, but the arithmetic is float (BBox.h:115-119).
+  double area() const {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    float a = 2 * (dx * dy + dy * dz + dx * dz);
+    return (double)a;
+  }
+};
+
+// Same 32-byte layout as DynBVH::BVHNode (Model/Groups/DynBVH.h:37-77): bounds @0,
+// child @24, axis @28, {isLeftCheaper bit0, isLargeSubtree bit1} @29, children @30.
+struct Node32 {
+  Box bounds;
+  int32_t child;
+  uint8_t axis;
+  uint8_t bits;
+  int16_t children;
+  bool isLeaf() const { return children != 0; }
+  bool leftCheaper() const { return bits & 1; }
+};
+static_assert(sizeof(Node32) == 32, "BVHNode is 32 bytes");
+
+struct Mesh {
+  const float* verts;  // 3 floats per vertex
+  int nverts;
+  const uint32_t* idx;  // 3 per triangle
+  int ntris;
+  const float* vtx(int tri, int k) const { return verts + 3 * (size_t)idx[3 * (size_t)tri + k]; }
+};
+
+// Mesh::getBBox (Mesh.cc:228-237) into a reset box (DynBVH.cc:1736-1737).
+inline Box triBox(const Mesh& m, int t) {
+  Box b;
+  const float* p1 = m.vtx(t, 0);
+  for (int i = 0; i < 3; ++i) {
+    b.lo[i] = p1[i];
+    b.hi[i] = p1[i];
+  }
+  b.extendPoint(m.vtx(t, 1));
+  b.extendPoint(m.vtx(t, 2));
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// SAH build (DynBVH.cc:1699-1857, 2277-2440)
+// ---------------------------------------------------------------------------
+struct Builder {
+  const Mesh& mesh;
+  std::vector<Node32>& nodes;
+  std::vector<int32_t>& ids;
+  std::vector<Box> obj_bounds;
+  std::vector<Vec> obj_centroids;
+  int next_free = 1;
+  int num_nodes = 0;
+
+  struct Partition {
+    Box bounds_l, bounds_r;
+    int axis;
+    int split;
+  };
+  struct Eval {
+    Box bounds_r;
+    float position;
+    float cost;
+    int num_left, num_right, event, axis;
+  };
+  struct Event {
+    float position;
+    int obj_id;
+    float left_area;
+  };
+
+  Builder(const Mesh& m, std::vector<Node32>& n, std::vector<int32_t>& i)
+      : mesh(m), nodes(n), ids(i) {}
+
+  // DynBVH.cc:2277-2308
+  Partition partition2(int nodeID, int objBegin) {
+    Partition p;
+    p.bounds_l.reset();
+    p.bounds_r.reset();
+    const float nodeArea = (float)nodes[nodeID].bounds.area();
+    const float leftArea = (float)obj_bounds[ids[objBegin]].area();
+    const float rightArea = (float)obj_bounds[ids[objBegin + 1]].area();
+    const float leafCost = kCIsec * 2;
+    const float newCost = kCTrav + (leftArea * kCIsec + rightArea * kCIsec) / nodeArea;
+    if (newCost < leafCost) {
+      const Box& nb = nodes[nodeID].bounds;
+      float d0 = nb.hi[0] - nb.lo[0], d1 = nb.hi[1] - nb.lo[1], d2 = nb.hi[2] - nb.lo[2];
+      // Vector::indexOfMaxComponent (Core/Geometry/Vector.h:382-393)
+      int axis = (d0 > d1) ? (d0 > d2 ? 0 : 2) : (d1 > d2 ? 1 : 2);
+      p.axis = axis;
+      p.split = objBegin + 1;
+      if (obj_bounds[ids[objBegin]].lo[axis] > obj_bounds[ids[objBegin + 1]].lo[axis])
+        std::swap(ids[objBegin], ids[objBegin + 1]);
+      p.bounds_l = obj_bounds[ids[objBegin]];
+      p.bounds_r = obj_bounds[ids[objBegin + 1]];
+      return p;
+    }
+    p.axis = -1;
+    return p;
+  }
+
+  void makeEvents(int first, int last, int axis, std::vector<Event>& ev) {
+    ev.clear();
+    ev.reserve(last - first);
+    for (int i = first; i < last; ++i) {
+      Event e;
+      e.position = obj_centroids[ids[i]].v[axis];
+      e.obj_id = ids[i];
+      e.left_area = 0.f;
+      ev.push_back(e);
+    }
+    std::sort(ev.begin(), ev.end(),
+              [](const Event& x, const Event& y) { return x.position < y.position; });
+  }
+
+  // DynBVH.cc:2372-2440
+  bool buildEvents(int parentID, int first, int last, int axis, Eval& best) {
+    std::vector<Event> ev;
+    makeEvents(first, last, axis, ev);
+    const int n = (int)ev.size();
+    Box left_box;
+    left_box.reset();
+    std::vector<int> num_left(n), num_right(n);
+    int nl = 0, nr = n;
+    for (int i = 0; i < n; ++i) {
+      num_left[i] = nl;
+      num_right[i] = nr;
+      ev[i].left_area = (float)left_box.area();
+      left_box.extendBox(obj_bounds[ev[i].obj_id]);
+      nl++;
+      nr--;
+    }
+    Box right_box;
+    right_box.reset();
+    best.cost = FLT_MAX;
+    best.event = -1;
+    const float inv_parent_area = (float)(1.0f / nodes[parentID].bounds.area());
+    for (int i = n - 1; i >= 0; --i) {
+      right_box.extendBox(obj_bounds[ev[i].obj_id]);
+      if (num_left[i] > 0 && num_right[i] > 0) {
+        const float right_area = (float)right_box.area();
+        float cost = (num_left[i] * ev[i].left_area + num_right[i] * right_area);
+        cost *= inv_parent_area;
+        cost *= kCIsec;
+        cost += kCTrav;
+        if (cost < best.cost) {
+          best.bounds_r = right_box;
+          best.cost = cost;
+          best.position = ev[i].position;
+          best.axis = axis;
+          best.event = i;
+          best.num_left = num_left[i];
+          best.num_right = num_right[i];
+        }
+      }
+    }
+    return best.event != -1;
+  }
+
+  // DynBVH.cc:2310-2370
+  Partition partitionSAH(int nodeID, int objBegin, int objEnd) {
+    Partition p;
+    p.bounds_l.reset();
+    p.bounds_r.reset();
+    p.split = 0;
+    const int n = objEnd - objBegin;
+    if (n == 1) {
+      p.axis = -1;
+      return p;
+    }
+    Eval best;
+    best.cost = kCIsec * n;
+    best.axis = -1;
+    best.position = FLT_MAX;
+    best.event = -1;
+    best.bounds_r.reset();
+    if (n == 2) return partition2(nodeID, objBegin);
+    for (int axis = 0; axis < 3; ++axis) {
+      Eval e;
+      if (buildEvents(nodeID, objBegin, objEnd, axis, e)) {
+        if (e.cost < best.cost) best = e;
+      }
+    }
+    p.bounds_r = best.bounds_r;
+    p.axis = best.axis;
+    if (p.axis != -1) {
+      std::vector<Event> ev;
+      makeEvents(objBegin, objEnd, p.axis, ev);
+      p.split = objBegin + best.event;
+      for (int i = objBegin; i < objEnd; ++i) {
+        ids[i] = ev[i - objBegin].obj_id;
+        if (i < p.split) p.bounds_l.extendBox(obj_bounds[ids[i]]);
+      }
+    }
+    return p;
+  }
+
+  // DynBVH.cc:1801-1857 (exact build, no approximate path, no rotation costs)
+  void build(int nodeID, int objBegin, int objEnd) {
+    Partition p = partitionSAH(nodeID, objBegin, objEnd);
+    Node32& node = nodes[nodeID];
+    if (p.axis == -1) {
+      node.child = objBegin;
+      node.children = (int16_t)(objEnd - objBegin);
+      node.axis = 0;
+      std::sort(ids.begin() + objBegin, ids.begin() + objEnd);
+      num_nodes++;
+    } else {
+      int my_spot = next_free;
+      next_free += 2;
+      node.child = my_spot;
+      node.children = 0;
+      node.axis = (uint8_t)p.axis;
+      num_nodes++;
+      nodes[my_spot + 0].bounds = p.bounds_l;
+      nodes[my_spot + 1].bounds = p.bounds_r;
+      build(my_spot + 0, objBegin, p.split);
+      build(my_spot + 1, p.split, objEnd);
+    }
+  }
+
+  // DynBVH.cc:1699-1793 with numProcs == 1
+  void rebuild() {
+    const int N = mesh.ntris;
+    nodes.assign(2 * (size_t)N, Node32());
+    for (auto& nd : nodes) {
+      nd.bounds.reset();
+      nd.child = 0;
+      nd.axis = 0;
+      nd.bits = 0;
+      nd.children = 0;
+    }
+    ids.resize(N);
+    obj_bounds.resize(N);
+    obj_centroids.resize(N);
+    // Mesh::computeBounds: box of all vertices (Mesh.cc:195-225)
+    Box root;
+    root.reset();
+    for (int v = 0; v < mesh.nverts; ++v) root.extendPoint(mesh.verts + 3 * (size_t)v);
+    nodes[0].bounds = root;
+    for (int i = 0; i < N; ++i) {
+      ids[i] = i;
+      obj_bounds[i] = triBox(mesh, i);
+      // BBox::center = Interpolate(min, max, 0.5f) (BBox.h:63-65, MiscMath.h:136-139)
+      for (int a = 0; a < 3; ++a)
+        obj_centroids[i].v[a] = obj_bounds[i].hi[a] * 0.5f + obj_bounds[i].lo[a] * (1 - 0.5f);
+    }
+    num_nodes = 0;
+    next_free = 1;
+    build(0, 0, N);
+    // computeSubTreeSizes with largeSubtreeSize = nodes.size() / 20 (DynBVH.cc:1776, 1971-1996)
+    large_subtree = (unsigned)(nodes.size() / 20);
+    subtreeSizes(0);
+  }
+
+  unsigned large_subtree = 0;
+  unsigned subtreeSizes(int id) {
+    Node32& nd = nodes[id];
+    if (nd.isLeaf()) {
+      nd.bits &= ~2;
+      return 1;
+    }
+    const unsigned size = subtreeSizes(nd.child) + subtreeSizes(nd.child + 1) + 1;
+    if (size > large_subtree)
+      nodes[id].bits |= 2;
+    else
+      nodes[id].bits &= ~2;
+    return size;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// RTSAH (DynBVH.cc:1594-1678, TreeTraversalProbability.{h,cc})
+// ---------------------------------------------------------------------------
+struct IcsiLog {  // Core/Math/ExponSSE.h:20-73, precision 15
+  int n = 15;
+  std::vector<float> table;
+  IcsiLog() {
+    table.resize(1 << n);
+    float oneToTwo = 1.0f + (1.0f / (float)(1 << (n + 1)));
+    for (int i = 0; i < (1 << n); ++i) {
+      table[i] = logf(oneToTwo) / 0.69314718055995f;
+      oneToTwo += 1.0f / (float)(1 << n);
+    }
+  }
+  float log(float val) const {
+    int32_t bits;
+    std::memcpy(&bits, &val, 4);
+    const int e = ((bits >> 23) & 255) - 127;
+    const int man = (bits & 0x7FFFFF) >> (23 - n);
+    return ((float)(e) + table[man]) * 0.69314718055995f;
+  }
+};
+
+inline float fastAtan2f(float y, float x) {  // Core/Math/Trig.h:153-178
+  const float kPi2 = (float)M_PI_2, kPi = (float)M_PI;
+  if (x == 0.0f) {
+    if (y > 0.0f) return kPi2;
+    if (y == 0.0f) return 0.0f;
+    return -kPi2;
+  }
+  float atan;
+  float z = y / x;
+  if (fabsf(z) < 1.0f) {
+    atan = z / (1.0f + 0.28f * z * z);
+    if (x < 0.0f) {
+      if (y < 0.0f) return atan - kPi;
+      return atan + kPi;
+    }
+  } else {
+    atan = kPi2 - z / (z * z + 0.28f);
+    if (y < 0.0f) return atan - kPi;
+  }
+  return atan;
+}
+
+struct TTP {
+  IcsiLog L;
+  static float altSign(int i) { return i % 2 == 1 ? -1.0f : 1.0f; }
+  float perpG(float v, float z, float x, float y, float z0, float x1) const {
+    const float a = y - v, b = z0 - z, c = x - x1;
+    const float aa = a * a, cc = c * c, bb = b * b;
+    const float s = sqrtf(cc + bb);
+    return a * s * fastAtan2f(a, s) + .25f * (aa - bb - cc) * L.log(aa + bb + cc);
+  }
+  static int otherIndex(unsigned a, unsigned b) { return (~(a + b)) & 3; }
+  float perpF(const Box& b0, const Box& b1, int d1, int d2) const {
+    const int axis = otherIndex(d1, d2);
+    const float v[2] = {b1.lo[axis], b1.hi[axis]};
+    const float y[2] = {b0.lo[axis], b0.hi[axis]};
+    const float x[2] = {b0.lo[d2], b0.hi[d2]};
+    const float z[2] = {b1.lo[d1], b1.hi[d1]};
+    const float x1 = b1.lo[d2];
+    const float z0 = b0.lo[d1];
+    float F = 0;
+    for (int i = 1; i <= 2; ++i)
+      for (int j = 1; j <= 2; ++j)
+        for (int k = 1; k <= 2; ++k)
+          for (int m = 1; m <= 2; ++m)
+            F += perpG(v[i - 1], z[j - 1], x[k - 1], y[m - 1], z0, x1) * altSign(i + j + k + m);
+    F *= (float)M_1_PI;
+    return fabsf(F);
+  }
+  float parG(float u, float v, float x, float y, float zi) const {
+    const float a = (x - u) * zi, b = (y - v) * zi;
+    const float aa = a * a, bb = b * b;
+    const float s1 = sqrtf(1 + aa), s2 = sqrtf(1 + bb);
+    return b * s1 * fastAtan2f(b, s1) + a * s2 * fastAtan2f(a, s2) - 0.5f * L.log(1 + aa + bb);
+  }
+  float parF(const Box& b0, const Box& b1, int flat) const {
+    const int d1 = (flat + 1) % 3, d2 = (flat + 2) % 3;
+    const float y[2] = {b0.lo[d1], b0.hi[d1]};
+    const float v[2] = {b1.lo[d1], b1.hi[d1]};
+    const float x[2] = {b0.lo[d2], b0.hi[d2]};
+    const float u[2] = {b1.lo[d2], b1.hi[d2]};
+    const float z = b0.lo[flat] - b1.lo[flat];
+    const float zi = 1.0f / z;
+    float F = 0;
+    for (int i = 1; i <= 2; ++i)
+      for (int j = 1; j <= 2; ++j)
+        for (int k = 1; k <= 2; ++k)
+          for (int m = 1; m <= 2; ++m)
+            F += parG(u[i - 1], v[j - 1], x[k - 1], y[m - 1], zi) * altSign(i + j + k + m);
+    F *= z * z * (float)M_1_PI;
+    return fabsf(F);
+  }
+  static float overlap(const Box& a, const Box& b) {
+    Box o = a;
+    for (int i = 0; i < 3; ++i) {
+      o.lo[i] = fmax_(o.lo[i], b.lo[i]);
+      o.hi[i] = fmin_(o.hi[i], b.hi[i]);
+    }
+    float d0 = o.hi[0] - o.lo[0], d1 = o.hi[1] - o.lo[1], d2 = o.hi[2] - o.lo[2];
+    if (d0 < 0 || d1 < 0 || d2 < 0) return 0;
+    return (float)o.area();
+  }
+  // TreeTraversalProbability.cc:3-95
+  void probs(const Box& parent, const Box& bL, const Box& bR, float& lr, float& jl,
+             float& jr) const {
+    float dl[3] = {bL.hi[0] - bL.lo[0], bL.hi[1] - bL.lo[1], bL.hi[2] - bL.lo[2]};
+    float dr[3] = {bR.hi[0] - bR.lo[0], bR.hi[1] - bR.lo[1], bR.hi[2] - bR.lo[2]};
+    const bool Linf[3] = {dl[1] == 0 || dl[2] == 0, dl[0] == 0 || dl[2] == 0,
+                          dl[0] == 0 || dl[1] == 0};
+    const bool Rinf[3] = {dr[1] == 0 || dr[2] == 0, dr[0] == 0 || dr[2] == 0,
+                          dr[0] == 0 || dr[1] == 0};
+    lr = 0;
+    for (int i = 0; i < 6; ++i) {
+      const int ii = i % 3;
+      if (Linf[ii]) continue;
+      if (i < 3) {
+        if (bL.hi[ii] >= bR.hi[ii]) continue;
+      } else {
+        if (bL.lo[ii] <= bR.lo[ii]) continue;
+      }
+      for (int j = 0; j < 6; ++j) {
+        if (i == j) continue;
+        const int jj = j % 3;
+        if (Rinf[jj]) continue;
+        Box pL = bL;
+        if (i < 3)
+          pL.lo[ii] = pL.hi[ii];
+        else
+          pL.hi[ii] = pL.lo[ii];
+        Box pR = bR;
+        if (j < 3)
+          pR.lo[jj] = pR.hi[jj];
+        else
+          pR.hi[jj] = pR.lo[jj];
+        if (ii == jj) {
+          if (i < 3) {
+            if (pL.hi[ii] >= pR.lo[ii]) continue;
+            lr += parF(pL, pR, ii);
+          } else {
+            if (pL.lo[ii] <= pR.hi[ii]) continue;
+            lr += parF(pL, pR, ii);
+          }
+        } else {
+          if (j < 3) {
+            if (pL.hi[jj] <= pR.hi[jj]) continue;
+            pL.lo[jj] = fmax_(pL.lo[jj], pR.hi[jj]);
+          } else {
+            if (pL.lo[jj] >= pR.lo[jj]) continue;
+            pL.hi[jj] = fmin_(pL.hi[jj], pR.lo[jj]);
+          }
+          if (i < 3) {
+            pR.lo[ii] = fmax_(pR.lo[ii], pL.hi[ii]);
+            lr += perpF(pL, pR, ii, jj);
+          } else {
+            pR.hi[ii] = fmin_(pR.hi[ii], pL.lo[ii]);
+            lr += perpF(pL, pR, ii, jj);
+          }
+        }
+      }
+    }
+    const float pinv = (float)(1.0f / parent.area());
+    lr += overlap(bL, bR);
+    lr *= pinv;
+    // computeArea() is double, so the product and the subtraction are in double.
+    jl = (float)(bL.area() * (double)pinv - (double)lr);
+    jr = (float)(bR.area() * (double)pinv - (double)lr);
+  }
+};
+
+struct Cost2 {
+  float c[2];
+};
+
+// DynBVH.cc:1594-1678 with opaque materials (canAttenuateShadows() == false).
+Cost2 subtreeCost(std::vector<Node32>& nodes, const TTP& ttp, int id) {
+  const float ISEC = 3, TRAV = 1;
+  Node32& node = nodes[id];
+  if (node.isLeaf()) {
+    Cost2 r;
+    r.c[0] = node.children * ISEC + TRAV;
+    r.c[1] = 0;
+    return r;
+  }
+  float Plr, Pjl, Pjr;
+  ttp.probs(nodes[id].bounds, nodes[node.child].bounds, nodes[node.child + 1].bounds, Plr, Pjl,
+            Pjr);
+  const float Pl = Pjl + Plr;
+  const float Pr = Pjr + Plr;
+  Cost2 Cl = subtreeCost(nodes, ttp, node.child);
+  Cost2 Cr = subtreeCost(nodes, ttp, node.child + 1);
+  const float Pe = 1 - Pjr - Pjl - Plr;
+  const float lCost = Pl * Cl.c[0] + (Pjr + Plr * Cl.c[1]) * Cr.c[0] + (1 + Pl * Cl.c[1] + (1 - Pl)) * TRAV;
+  const float rCost = Pr * Cr.c[0] + (Pjl + Plr * Cr.c[1]) * Cl.c[0] + (1 + Pr * Cr.c[1] + (1 - Pr)) * TRAV;
+  Cost2 cost;
+  cost.c[0] = fmin_(lCost, rCost);
+  float c1 = Pjl * Cl.c[1] + Pjr * Cr.c[1] + Plr * Cl.c[1] * Cr.c[1] + Pe;
+  c1 = c1 <= 0.0f ? 0.0f : (c1 >= 1.0f ? 1.0f : c1);  // Clamp (Core/Math/MiscMath.h:82-85)
+  cost.c[1] = c1;
+  if (lCost < rCost)
+    node.bits |= 1;
+  else
+    node.bits &= ~1;
+  return cost;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle tests
+// ---------------------------------------------------------------------------
+// RayPacketData, reduced to the fields the path touches (Interface/RayPacket.h:58-130).
+// state: 0 = not hit (hitMatl == 0), 1 = hit (hitMatl = mesh material), -1 = masked
+// (hitMatl == ~0, RayPacket.h:488-502).
+struct Packet {
+  float org[3][64];
+  float dir[3][64];
+  float inv[3][64];
+  float minT[64];
+  int state[64];
+  int prim[64];
+  float u[64], v[64];
+  bool anyHit;
+};
+
+inline bool masked(const Packet& p, int i) { return p.state[i] == -1; }
+inline bool wasHit(const Packet& p, int i) { return p.state[i] != 0; }
+
+// RayPacket::hit (RayPacket.h:531-542)
+inline bool packetHit(Packet& p, int i, float t, int prim) {
+  if (t > kTEps && t < p.minT[i]) {
+    p.minT[i] = t;
+    p.state[i] = 1;
+    p.prim[i] = prim;
+    return true;
+  }
+  return false;
+}
+
+// Per-triangle Wald data (WaldTriangle.cc:64-127)
+struct Wald {
+  float n_u, n_v, n_d;
+  int k;
+  float b_nu, b_nv, b_d, n_k, c_nu, c_nv, c_d;
+};
+
+Wald waldSetup(const float* P1, const float* P2, const float* P3) {
+  Vec p1, p2, p3;
+  for (int i = 0; i < 3; ++i) {
+    p1.v[i] = P1[i];
+    p2.v[i] = P2[i];
+    p3.v[i] = P3[i];
+  }
+  Vec nrm = vcross(vsub(p2, p1), vsub(p3, p1));
+  // Vector::normalize (Core/Geometry/Vector.h:322-331)
+  float l = sqrtf(nrm.v[0] * nrm.v[0] + nrm.v[1] * nrm.v[1] + nrm.v[2] * nrm.v[2]);
+  float scale = 1 / l;
+  nrm.v[0] *= scale;
+  nrm.v[1] *= scale;
+  nrm.v[2] *= scale;
+  unsigned n = 0, u, v;
+  if (fabsf(nrm.v[1]) > fabsf(nrm.v[n])) n = 1;
+  if (fabsf(nrm.v[2]) > fabsf(nrm.v[n])) n = 2;
+  switch (n) {
+    case 0: u = 1; v = 2; break;
+    case 1: u = 2; v = 0; break;
+    default: u = 0; v = 1; break;
+  }
+  Wald w;
+  w.k = (int)n;
+  float inv_nk = 1 / nrm.v[n];
+  w.n_u = nrm.v[u] * inv_nk;
+  w.n_v = nrm.v[v] * inv_nk;
+  w.n_d = p1.v[u] * w.n_u + p1.v[v] * w.n_v + p1.v[n];
+  float s;
+  w.c_nu = +p2.v[v] - p1.v[v];
+  w.c_nv = -p2.v[u] + p1.v[u];
+  w.c_d = -(p1.v[u] * w.c_nu + p1.v[v] * w.c_nv);
+  s = 1.f / (p3.v[u] * w.c_nu + p3.v[v] * w.c_nv + w.c_d);
+  w.c_nu *= s;
+  w.c_nv *= s;
+  w.c_d *= s;
+  w.b_nu = +p3.v[v] - p1.v[v];
+  w.b_nv = -p3.v[u] + p1.v[u];
+  w.b_d = -(p1.v[u] * w.b_nu + p1.v[v] * w.b_nv);
+  s = 1.f / (p2.v[u] * w.b_nu + p2.v[v] * w.b_nv + w.b_d);
+  w.b_nu *= s;
+  w.b_nv *= s;
+  w.b_d *= s;
+  w.n_k = nrm.v[n];
+  return w;
+}
+
+struct Scene {
+  Mesh mesh;
+  const Node32* nodes;
+  const int32_t* ids;
+  int prim_kind;  // 0 = Kensler-Shirley, 1 = Wald
+  std::vector<Wald> wald;
+};
+
+// KenslerShirleyTriangle::intersect, scalar branch (KenslerShirleyTriangle.cc:336-377)
+void intersectKS(const Scene& sc, int tri, Packet& p, int b, int e) {
+  Vec p0, p1, p2;
+  for (int i = 0; i < 3; ++i) {
+    p0.v[i] = sc.mesh.vtx(tri, 0)[i];
+    p1.v[i] = sc.mesh.vtx(tri, 1)[i];
+    p2.v[i] = sc.mesh.vtx(tri, 2)[i];
+  }
+  const Vec edge0 = vsub(p1, p0);
+  const Vec edge1 = vsub(p0, p2);
+  const Vec normal = vcross(edge0, edge1);
+  for (int r = b; r < e; ++r) {
+    Vec o, d;
+    for (int i = 0; i < 3; ++i) {
+      o.v[i] = p.org[i][r];
+      d.v[i] = p.dir[i][r];
+    }
+    const float oldt = p.minT[r];
+    const float rcp = 1.0f / vdot(normal, d);
+    const Vec edge2 = vsub(p0, o);
+    const float toverd = vdot(normal, edge2) * rcp;
+    if (toverd > oldt - kTEps || toverd < kTEps) continue;
+    const Vec interm = vcross(edge2, d);
+    const float uoverd = vdot(interm, edge1) * rcp;
+    if (uoverd < 0.0f) continue;
+    const float voverd = vdot(interm, edge0) * rcp;
+    if (uoverd + voverd > 1.0f || voverd < 0.0f) continue;
+    if (packetHit(p, r, toverd, tri)) {
+      p.u[r] = uoverd;
+      p.v[r] = voverd;
+    }
+  }
+}
+
+// WaldTriangle::intersect, scalar branch (WaldTriangle.cc:489-555), per-ray origin.
+// (The ConstantOrigin shortcut computes the same f0 when all origins are equal.)
+void intersectWald(const Scene& sc, int tri, Packet& p, int b, int e) {
+  const Wald& w = sc.wald[tri];
+  const int axis = w.k;
+  const int ku = (w.k == 2) ? 0 : w.k + 1;
+  const int kv = (w.k == 0) ? 2 : w.k - 1;
+  for (int i = b; i < e; ++i) {
+    const float nd0 = w.n_u * p.dir[ku][i] + w.n_v * p.dir[kv][i] + p.dir[axis][i];
+    const float nd = 1.f / nd0;
+    const float org_k = p.org[axis][i], org_ku = p.org[ku][i], org_kv = p.org[kv][i];
+    const float f0 = w.n_d - (org_k + w.n_u * org_ku + w.n_v * org_kv);
+    const float f = f0 * nd;
+    if (f < kTEps || f > p.minT[i]) continue;
+    const float hu = org_ku + f * p.dir[ku][i];
+    const float hv = org_kv + f * p.dir[kv][i];
+    const float lambda = w.b_d + hu * w.b_nu + hv * w.b_nv;
+    if (lambda < 0.f) continue;
+    const float mue = w.c_d + hu * w.c_nu + hv * w.c_nv;
+    if (mue < 0.f || mue + lambda > 1.f) continue;
+    if (packetHit(p, i, f, tri)) {
+      p.u[i] = lambda;
+      p.v[i] = mue;
+    }
+  }
+}
+
+inline void intersectPrim(const Scene& sc, int tri, Packet& p, int b, int e) {
+  if (sc.prim_kind == 1)
+    intersectWald(sc, tri, p, b, e);
+  else
+    intersectKS(sc, tri, p, b, e);
+}
+
+// ---------------------------------------------------------------------------
+// Packet traversal (DynBVH.cc:101-404, scalar firstIntersects 655-725, lastIntersects 882-905)
+// ---------------------------------------------------------------------------
+struct IA {
+  float min_rcp[3], max_rcp[3], min_org[3], max_org[3];
+};
+
+struct Counters {
+  int64_t box_tests = 0;  // per-ray slab evaluations (single-ray traversal)
+  int64_t tri_tests = 0;  // per-ray triangle tests
+};
+
+inline bool slab(const Box& box, const Packet& p, int i) {
+  float tmin = kTEps;
+  float tmax = p.minT[i];
+  for (int c = 0; c < 3; ++c) {
+    float t0 = (box.lo[c] - p.org[c][i]) * p.inv[c][i];
+    float t1 = (box.hi[c] - p.org[c][i]) * p.inv[c][i];
+    float near_ = (t0 < t1) ? t0 : t1;
+    float far_ = (t0 < t1) ? t1 : t0;
+    tmin = (tmin < near_) ? near_ : tmin;
+    tmax = (far_ < tmax) ? far_ : tmax;
+  }
+  return tmin <= tmax;
+}
+
+int firstIntersects(const Box& box, const Packet& p, int rb, int re, const IA& ia,
+                    Counters* cnt) {
+  int rayBegin = rb;
+  for (; rayBegin < re; rayBegin++) {
+    if (masked(p, rayBegin)) continue;
+    if (p.anyHit && wasHit(p, rayBegin)) continue;
+    break;
+  }
+  for (int i = rayBegin; i < re; ++i) {
+    if (masked(p, i)) continue;
+    if (p.anyHit && wasHit(p, i)) continue;
+    if (cnt) cnt->box_tests++;
+    if (slab(box, p, i)) return i;
+    if (i == rayBegin) {
+      float tmin_f = (float)1e-5;
+      float tmax_f = FLT_MAX;
+      for (int axis = 0; axis < 3; ++axis) {
+        float a = box.lo[axis] - ia.max_org[axis];
+        float b = box.hi[axis] - ia.min_org[axis];
+        float ar0 = a * ia.min_rcp[axis];
+        float ar1 = a * ia.max_rcp[axis];
+        float br0 = b * ia.min_rcp[axis];
+        float br1 = b * ia.max_rcp[axis];
+        a = (br0 < br1) ? br0 : br1;
+        a = (a < ar1) ? a : ar1;
+        a = (a < ar0) ? a : ar0;
+        b = (br0 < br1) ? br1 : br0;
+        b = (b < ar1) ? ar1 : b;
+        b = (b < ar0) ? ar0 : b;
+        tmin_f = (tmin_f < a) ? a : tmin_f;
+        tmax_f = (tmax_f > b) ? b : tmax_f;
+      }
+      if (tmin_f > tmax_f) return re;
+    }
+  }
+  return re;
+}
+
+int lastIntersects(const Box& box, const Packet& p, int rb, int re) {
+  int rayEnd = re;
+  for (; rb < rayEnd; rayEnd--) {
+    if (masked(p, rayEnd - 1)) continue;
+    if (p.anyHit && wasHit(p, rayEnd - 1)) continue;
+    break;
+  }
+  for (int i = rayEnd - 1; i > rb; --i) {
+    if (p.anyHit && wasHit(p, i)) continue;
+    if (slab(box, p, i)) return i;
+  }
+  return rb;
+}
+
+void intersectNode(const Scene& sc, int nodeID, Packet& p, int rb, int re, const IA& ia,
+                   Counters* cnt) {
+  const Node32& node = sc.nodes[nodeID];
+  int firstActive = firstIntersects(node.bounds, p, rb, re, ia, cnt);
+  if (firstActive == re) return;
+  if (node.isLeaf()) {
+    int lastActive = lastIntersects(node.bounds, p, rb, re);
+    int sb = firstActive, se = lastActive + 1;
+    for (int k = 0; k < node.children; ++k) {
+      const int object_id = sc.ids[node.child + k];
+      if (cnt) cnt->tri_tests += se - sb;
+      if (p.anyHit) {
+        // DynBVH.cc:315-375; every material is opaque, so attenuateShadows turns each
+        // newly hit ray black and the ray's minT is overridden with T_EPSILON.
+        intersectPrim(sc, object_id, p, sb, se);
+        bool terminated = false;
+        for (int i = sb; i < se; ++i) {
+          if (!wasHit(p, i) || masked(p, i)) continue;
+          terminated = true;
+          p.minT[i] = kTEps;
+        }
+        if (terminated) {
+          int start, end;
+          for (start = sb; start < se; ++start)
+            if (!wasHit(p, start) && !masked(p, start)) break;
+          for (end = se - 1; end > start; --end)
+            if (!wasHit(p, end) && !masked(p, end)) break;
+          sb = start;
+          se = end + 1;
+          if (end < start) break;
+        }
+      } else {
+        intersectPrim(sc, object_id, p, sb, se);
+      }
+    }
+  } else {
+    int front;
+    if (p.anyHit)
+      front = node.leftCheaper() ? 0 : 1;
+    else
+      front = p.dir[node.axis][firstActive] > 0 ? 0 : 1;
+    intersectNode(sc, node.child + front, p, firstActive, re, ia, cnt);
+    intersectNode(sc, node.child + 1 - front, p, firstActive, re, ia, cnt);
+  }
+}
+
+// DynBVH::intersect (DynBVH.cc:101-272), scalar branch
+void intersectPacket(const Scene& sc, Packet& p, int n, Counters* cnt) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 3; ++j) p.inv[j][i] = (float)1 / p.dir[j][i];
+  IA ia;
+  for (int a = 0; a < 3; ++a) {
+    ia.min_rcp[a] = FLT_MAX;
+    ia.max_rcp[a] = -FLT_MAX;
+    ia.min_org[a] = FLT_MAX;
+    ia.max_org[a] = -FLT_MAX;
+  }
+  for (int r = 0; r < n; ++r) {
+    if (masked(p, r)) continue;
+    for (int a = 0; a < 3; ++a) {
+      const float rc = p.inv[a][r], o = p.org[a][r];
+      ia.min_rcp[a] = (ia.min_rcp[a] < rc) ? ia.min_rcp[a] : rc;
+      ia.max_rcp[a] = (ia.max_rcp[a] < rc) ? rc : ia.max_rcp[a];
+      ia.min_org[a] = (ia.min_org[a] < o) ? ia.min_org[a] : o;
+      ia.max_org[a] = (ia.max_org[a] < o) ? o : ia.max_org[a];
+    }
+  }
+  intersectNode(sc, 0, p, 0, n, ia, cnt);
+}
+
+struct RayIO {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+  const uint8_t* masked;  // may be null
+  int32_t* prim;
+  float *t, *u, *v;
+  uint8_t* occluded;  // may be null
+};
+
+void tracePacketRange(const Scene& sc, const RayIO& io, int64_t first, int n, bool anyHit,
+                      Counters* cnt) {
+  Packet p;
+  p.anyHit = anyHit;
+  for (int i = 0; i < n; ++i) {
+    const int64_t r = first + i;
+    p.org[0][i] = io.ox[r];
+    p.org[1][i] = io.oy[r];
+    p.org[2][i] = io.oz[r];
+    p.dir[0][i] = io.dx[r];
+    p.dir[1][i] = io.dy[r];
+    p.dir[2][i] = io.dz[r];
+    p.prim[i] = -1;
+    p.u[i] = p.v[i] = 0.f;
+    if (io.masked && io.masked[r]) {
+      p.state[i] = -1;          // RayPacket::maskRay
+      p.minT[i] = -kMaxT;
+    } else {
+      p.state[i] = 0;
+      p.minT[i] = io.tmax[r];
+    }
+  }
+  intersectPacket(sc, p, n, cnt);
+  for (int i = 0; i < n; ++i) {
+    const int64_t r = first + i;
+    if (io.prim) io.prim[r] = (p.state[i] == 1) ? p.prim[i] : (p.state[i] == -1 ? -2 : -1);
+    if (io.t) io.t[r] = p.minT[i];
+    if (io.u) io.u[r] = p.u[i];
+    if (io.v) io.v[r] = p.v[i];
+    if (io.occluded) io.occluded[r] = wasHit(p, i) ? 1 : 0;
+  }
+}
+
+template <class F>
+void parallelFor(int64_t n, int nthreads, F f) {
+  if (nthreads <= 1 || n <= 1) {
+    for (int64_t i = 0; i < n; ++i) f(i, 0);
+    return;
+  }
+  std::atomic<int64_t> next(0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t)
+    th.emplace_back([&, t]() {
+      for (;;) {
+        int64_t i = next.fetch_add(1);
+        if (i >= n) break;
+        f(i, t);
+      }
+    });
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_node_size(void) { return (int)sizeof(Node32); }
+
+// Builds a DynBVH over the mesh exactly as DynBVH::rebuild(0,1) does, then runs the RTSAH
+// pass when rtsah != 0. nodes_out must hold 2*ntris 32-byte nodes, ids_out ntris ints.
+// Returns the number of nodes in use (num_nodes).
+int oracle_build(const float* verts, int nverts, const uint32_t* idx, int ntris, int rtsah,
+                 void* nodes_out, int32_t* ids_out) {
+  if (ntris <= 0) return -1;
+  Mesh m{verts, nverts, idx, ntris};
+  std::vector<Node32> nodes;
+  std::vector<int32_t> ids;
+  Builder b(m, nodes, ids);
+  b.rebuild();
+  if (rtsah) {
+    TTP ttp;
+    subtreeCost(nodes, ttp, 0);
+  }
+  std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(Node32));
+  std::memcpy(ids_out, ids.data(), ids.size() * sizeof(int32_t));
+  return b.num_nodes;
+}
+
+// Recomputes isLeftCheaper on an existing tree (DynBVH::computeTraversalCost).
+void oracle_rtsah(void* nodes, int n_nodes) {
+  std::vector<Node32> nv((Node32*)nodes, (Node32*)nodes + n_nodes);
+  TTP ttp;
+  subtreeCost(nv, ttp, 0);
+  std::memcpy(nodes, nv.data(), nv.size() * sizeof(Node32));
+}
+
+// Traces n rays through the tree in packets of packet_size (1..64) consecutive rays.
+// prim: triangle id, -1 no hit, -2 masked. t: final minT. occluded: RayPacket::wasHit.
+// counters (optional, int64[2]): total box tests and triangle tests.
+int oracle_trace(const float* verts, int nverts, const uint32_t* idx, int ntris,
+                 const void* nodes, const int32_t* ids, int prim_kind, const float* ox,
+                 const float* oy, const float* oz, const float* dx, const float* dy,
+                 const float* dz, const float* tmax, const uint8_t* maskd, int64_t n,
+                 int packet_size, int any_hit, int nthreads, int32_t* prim, float* t,
+                 float* u, float* v, uint8_t* occluded, int64_t* counters) {
+  if (packet_size < 1 || packet_size > 64) return -1;
+  Scene sc;
+  sc.mesh = Mesh{verts, nverts, idx, ntris};
+  sc.nodes = (const Node32*)nodes;
+  sc.ids = ids;
+  sc.prim_kind = prim_kind;
+  if (prim_kind == 1) {
+    sc.wald.resize(ntris);
+    for (int i = 0; i < ntris; ++i)
+      sc.wald[i] = waldSetup(sc.mesh.vtx(i, 0), sc.mesh.vtx(i, 1), sc.mesh.vtx(i, 2));
+  }
+  RayIO io{ox, oy, oz, dx, dy, dz, tmax, maskd, prim, t, u, v, occluded};
+  const int64_t npk = (n + packet_size - 1) / packet_size;
+  int nt = nthreads < 1 ? 1 : nthreads;
+  std::vector<Counters> per(nt);
+  parallelFor(npk, nt, [&](int64_t k, int tid) {
+    int64_t first = k * packet_size;
+    int cnt = (int)std::min<int64_t>(packet_size, n - first);
+    tracePacketRange(sc, io, first, cnt, any_hit != 0, counters ? &per[tid] : nullptr);
+  });
+  if (counters) {
+    counters[0] = counters[1] = 0;
+    for (auto& c : per) {
+      counters[0] += c.box_tests;
+      counters[1] += c.tri_tests;
+    }
+  }
+  return 0;
+}
+
+// Brute force over all triangles in ascending id order with the same triangle test and
+// RayPacket::hit rule; no tree. Used to pin the traversal.
+int oracle_bruteforce(const float* verts, int nverts, const uint32_t* idx, int ntris,
+                      int prim_kind, const float* ox, const float* oy, const float* oz,
+                      const float* dx, const float* dy, const float* dz, const float* tmax,
+                      const uint8_t* maskd, int64_t n, int any_hit, int nthreads, int32_t* prim,
+                      float* t, float* u, float* v, uint8_t* occluded) {
+  Scene sc;
+  sc.mesh = Mesh{verts, nverts, idx, ntris};
+  sc.nodes = nullptr;
+  sc.ids = nullptr;
+  sc.prim_kind = prim_kind;
+  if (prim_kind == 1) {
+    sc.wald.resize(ntris);
+    for (int i = 0; i < ntris; ++i)
+      sc.wald[i] = waldSetup(sc.mesh.vtx(i, 0), sc.mesh.vtx(i, 1), sc.mesh.vtx(i, 2));
+  }
+  parallelFor(n, nthreads < 1 ? 1 : nthreads, [&](int64_t r, int) {
+    Packet p;
+    p.anyHit = any_hit != 0;
+    p.org[0][0] = ox[r];
+    p.org[1][0] = oy[r];
+    p.org[2][0] = oz[r];
+    p.dir[0][0] = dx[r];
+    p.dir[1][0] = dy[r];
+    p.dir[2][0] = dz[r];
+    p.prim[0] = -1;
+    p.u[0] = p.v[0] = 0;
+    if (maskd && maskd[r]) {
+      p.state[0] = -1;
+      p.minT[0] = -kMaxT;
+    } else {
+      p.state[0] = 0;
+      p.minT[0] = tmax[r];
+    }
+    if (p.state[0] == 0) {
+      for (int tri = 0; tri < ntris; ++tri) {
+        intersectPrim(sc, tri, p, 0, 1);
+        if (p.anyHit && p.state[0] == 1) break;
+      }
+    }
+    if (prim) prim[r] = (p.state[0] == 1) ? p.prim[0] : (p.state[0] == -1 ? -2 : -1);
+    if (t) t[r] = p.minT[0];
+    if (u) u[r] = p.u[0];
+    if (v) v[r] = p.v[0];
+    if (occluded) occluded[r] = wasHit(p, 0) ? 1 : 0;
+  });
+  return 0;
+}
+
+// Refit without tree rotations: DynBVH::parallelUpdateBounds(numProcs == 1) ->
+// updateBounds<false>(0) (DynBVH.cc:2012-2123) on the current vertices, in place.
+void oracle_refit(const float* verts, int nverts, const uint32_t* idx, int ntris, void* nodes_v,
+                  const int32_t* ids) {
+  Mesh m{verts, nverts, idx, ntris};
+  Node32* nodes = (Node32*)nodes_v;
+  std::vector<Box> ob(ntris);
+  for (int i = 0; i < ntris; ++i) ob[i] = triBox(m, i);
+  struct R {
+    Node32* nodes;
+    const int32_t* ids;
+    const std::vector<Box>& ob;
+    void go(int id) {
+      Node32& node = nodes[id];
+      if (node.isLeaf()) {
+        node.bounds.reset();
+        for (int i = 0; i < node.children; ++i) node.bounds.extendBox(ob[ids[node.child + i]]);
+        node.axis = 0;
+      } else {
+        go(node.child);
+        go(node.child + 1);
+        node.bounds.reset();
+        node.bounds.extendBox(nodes[node.child].bounds);
+        node.bounds.extendBox(nodes[node.child + 1].bounds);
+      }
+    }
+  } rr{nodes, ids, ob};
+  rr.go(0);
+}
+
+// Wald per-triangle constants (11 floats + k as float) for tests of the setup pass.
+void oracle_wald_setup(const float* verts, const uint32_t* idx, int ntris, float* out12) {
+  Mesh m{verts, 0, idx, ntris};
+  for (int i = 0; i < ntris; ++i) {
+    Wald w = waldSetup(m.vtx(i, 0), m.vtx(i, 1), m.vtx(i, 2));
+    float* o = out12 + 12 * (size_t)i;
+    o[0] = w.n_u; o[1] = w.n_v; o[2] = w.n_d; o[3] = (float)w.k;
+    o[4] = w.b_nu; o[5] = w.b_nv; o[6] = w.b_d; o[7] = w.n_k;
+    o[8] = w.c_nu; o[9] = w.c_nv; o[10] = w.c_d; o[11] = 0.f;
+  }
+}
+
+}  // extern "C"

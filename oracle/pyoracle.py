@@ -1,0 +1,142 @@
+"""ctypes front end of the oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module, as
+the checker. It loads oracle/liboracle.so, built by oracle/Makefile.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+NODE_DTYPE = np.dtype([("lo", "<f4", (3,)), ("hi", "<f4", (3,)), ("child", "<i4"),
+                       ("axis", "u1"), ("bits", "u1"), ("children", "<i2")])
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        lib.oracle_build.restype = C.c_int
+        lib.oracle_build.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
+        lib.oracle_rtsah.restype = None
+        lib.oracle_rtsah.argtypes = [P, C.c_int]
+        lib.oracle_trace.restype = C.c_int
+        lib.oracle_trace.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_int] + [P] * 8 + [
+            C.c_int64, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P]
+        lib.oracle_bruteforce.restype = C.c_int
+        lib.oracle_bruteforce.argtypes = [P, C.c_int, P, C.c_int, C.c_int] + [P] * 8 + [
+            C.c_int64, C.c_int, C.c_int, P, P, P, P, P]
+        lib.oracle_refit.restype = None
+        lib.oracle_refit.argtypes = [P, C.c_int, P, C.c_int, P, P]
+        lib.oracle_wald_setup.restype = None
+        lib.oracle_wald_setup.argtypes = [P, P, C.c_int, P]
+        lib.oracle_node_size.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _mesh(verts, idx):
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    return v, i
+
+
+def build_bvh(verts, idx, rtsah: bool = True):
+    """DynBVH::rebuild(0,1) + RTSAH. Returns (nodes[2T] NODE_DTYPE, ids int32[T], num_nodes)."""
+    v, i = _mesh(verts, idx)
+    T = len(i)
+    nodes = np.zeros(2 * T, NODE_DTYPE)
+    ids = np.zeros(T, np.int32)
+    used = load().oracle_build(_p(v), len(v), _p(i), T, int(rtsah), _p(nodes), _p(ids))
+    if used < 0:
+        raise ValueError("oracle build failed")
+    return nodes, ids, used
+
+
+def rtsah(nodes):
+    load().oracle_rtsah(_p(nodes), len(nodes))
+
+
+def _rays(rays):
+    arr = {k: np.ascontiguousarray(rays[k], np.float32) for k in
+           ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
+    m = rays.get("masked")
+    m = None if m is None else np.ascontiguousarray(m, np.uint8)
+    return arr, m
+
+
+def trace(verts, idx, nodes, ids, rays, any_hit=False, packet_size=1, prim_kind=0,
+          threads=None, counters=False):
+    """Packet traversal over consecutive groups of packet_size rays."""
+    v, i = _mesh(verts, idx)
+    arr, m = _rays(rays)
+    n = len(arr["ox"])
+    out = {"prim": np.empty(n, np.int32), "t": np.empty(n, np.float32),
+           "u": np.empty(n, np.float32), "v": np.empty(n, np.float32),
+           "occluded": np.empty(n, np.uint8)}
+    cnt = np.zeros(2, np.int64) if counters else None
+    th = threads if threads is not None else (os.cpu_count() or 1)
+    rc = load().oracle_trace(_p(v), len(v), _p(i), len(i), _p(np.ascontiguousarray(nodes)),
+                             _p(np.ascontiguousarray(ids, np.int32)), prim_kind,
+                             *[_p(arr[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")],
+                             _p(m), n, packet_size, int(any_hit), th, _p(out["prim"]),
+                             _p(out["t"]), _p(out["u"]), _p(out["v"]), _p(out["occluded"]),
+                             _p(cnt))
+    if rc != 0:
+        raise ValueError("oracle trace failed")
+    if counters:
+        out["box_tests"] = int(cnt[0])
+        out["tri_tests"] = int(cnt[1])
+    return out
+
+
+def bruteforce(verts, idx, rays, any_hit=False, prim_kind=0, threads=None):
+    v, i = _mesh(verts, idx)
+    arr, m = _rays(rays)
+    n = len(arr["ox"])
+    out = {"prim": np.empty(n, np.int32), "t": np.empty(n, np.float32),
+           "u": np.empty(n, np.float32), "v": np.empty(n, np.float32),
+           "occluded": np.empty(n, np.uint8)}
+    th = threads if threads is not None else (os.cpu_count() or 1)
+    load().oracle_bruteforce(_p(v), len(v), _p(i), len(i), prim_kind,
+                             *[_p(arr[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")],
+                             _p(m), n, int(any_hit), th, _p(out["prim"]), _p(out["t"]),
+                             _p(out["u"]), _p(out["v"]), _p(out["occluded"]))
+    return out
+
+
+def refit(verts, idx, nodes, ids):
+    """updateBounds<false> without rotations, in place on nodes."""
+    v, i = _mesh(verts, idx)
+    assert nodes.flags["C_CONTIGUOUS"]
+    load().oracle_refit(_p(v), len(v), _p(i), len(i), _p(nodes),
+                        _p(np.ascontiguousarray(ids, np.int32)))
+
+
+def wald_setup(verts, idx):
+    v, i = _mesh(verts, idx)
+    out = np.empty((len(i), 12), np.float32)
+    load().oracle_wald_setup(_p(v), _p(i), len(i), _p(out))
+    return out

@@ -1,0 +1,123 @@
+"""ctypes bindings of libmh.so (include/mh.h).
+
+The library is the only compute path: if libmh.so is missing this module raises at
+import time instead of falling back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libmh.so"
+
+MH_OK = 0
+MH_CLOSEST_HIT = 0
+MH_ANY_HIT = 1
+MH_PRIM_KENSLER_SHIRLEY = 0
+MH_PRIM_WALD = 1
+MH_NO_HIT = -1
+MH_MASKED = -2
+T_EPSILON = np.float32(1.0e-4)
+MAXT = np.float32(1.0e19)
+
+# Reference BVHNode record (Model/Groups/DynBVH.h:37-77), 32 bytes.
+NODE_DTYPE = np.dtype([("lo", "<f4", (3,)), ("hi", "<f4", (3,)), ("child", "<i4"),
+                       ("axis", "u1"), ("bits", "u1"), ("children", "<i2")])
+assert NODE_DTYPE.itemsize == 32
+
+
+class MhError(RuntimeError):
+    """A libmh entry point returned a nonzero status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"libmh error {code}: {message}")
+        self.code = code
+
+
+class Rays(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax", "masked")]
+
+
+class Hits(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("prim", "t", "u", "v", "occluded")]
+
+
+class TraceOpts(C.Structure):
+    _fields_ = [("packet_size", C.c_int), ("stack_lds", C.c_int), ("blocks_per_cu", C.c_int),
+                ("counters", C.c_void_p)]
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int)]
+
+
+# Every symbol include/mh.h declares, with its ctypes signature.
+_P = C.c_void_p
+_SIGS = {
+    "mh_init": (C.c_int, [C.c_int]),
+    "mh_last_error": (C.c_char_p, []),
+    "mh_abi_version": (C.c_int, []),
+    "mh_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+    "mh_scene_create": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "mh_scene_destroy": (C.c_int, [_P]),
+    "mh_scene_build": (C.c_int, [_P, C.POINTER(BuildOpts)]),
+    "mh_dynbvh_build_host": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(BuildOpts),
+                                       _P, _P, C.POINTER(C.c_uint32)]),
+    "mh_scene_load_dynbvh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, C.c_int]),
+    "mh_scene_load_dynbvh_file": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "mh_scene_save_dynbvh_file": (C.c_int, [_P, C.c_char_p]),
+    "mh_scene_export_dynbvh": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), _P, C.POINTER(C.c_uint32)]),
+    "mh_scene_bounds": (C.c_int, [_P, _P, _P]),
+    "mh_scene_stats": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_size_t)]),
+    "mh_scene_set_vertices": (C.c_int, [_P, _P, C.c_int, _P]),
+    "mh_scene_refit": (C.c_int, [_P, _P]),
+    "mh_scene_vertices_device": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+    "mh_scene_displace_sine": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P]),
+    "mh_trace": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,
+                           C.POINTER(TraceOpts), _P]),
+    "mh_trace_host": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,
+                                C.POINTER(TraceOpts)]),
+    "mh_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "mh_free": (C.c_int, [_P]),
+    "mh_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
+    "mh_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+    "mh_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "mh_stream_destroy": (C.c_int, [_P]),
+    "mh_stream_sync": (C.c_int, [_P]),
+    "mh_time_trace": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,
+                                C.POINTER(TraceOpts), _P, C.c_int, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libmh.so once and attaches signatures; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m rfmanta_amd.build`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != MH_OK:
+        msg = load().mh_last_error()
+        raise MhError(code, msg.decode() if msg else "")
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data

@@ -1,0 +1,268 @@
+"""Python view of the C ABI: scenes, device buffers and tracing.
+
+Mirrors the reference's AccelerationStructure life cycle (Interface/AccelerationStructure.h:
+11-66): create over a mesh (setGroup), build (rebuild with group_changed), refit (update),
+load/save the DynBVH binary tree (buildFromFile/saveToFile) and intersect
+(Object::intersect, Interface/Object.h:34) over batches of rays in SoA form.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")
+
+
+class DeviceArray:
+    """One device allocation (mh_malloc) with numpy-style upload/download."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        L.check(L.load().mh_malloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes:
+            L.check(L.load().mh_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            L.check(L.load().mh_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        L.check(L.load().mh_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            L.check(L.load().mh_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class DeviceRays:
+    """SoA rays resident in device memory."""
+
+    n: int
+    arrays: dict
+
+    @classmethod
+    def upload(cls, rays: dict) -> "DeviceRays":
+        n = len(rays["ox"])
+        arrs = {k: DeviceArray.from_numpy(np.asarray(rays[k], np.float32)) for k in RAY_FIELDS}
+        if rays.get("masked") is not None:
+            arrs["masked"] = DeviceArray.from_numpy(np.asarray(rays["masked"], np.uint8))
+        return cls(n, arrs)
+
+    def struct(self) -> L.Rays:
+        r = L.Rays()
+        for k in RAY_FIELDS:
+            setattr(r, k, self.arrays[k].ptr)
+        r.masked = self.arrays["masked"].ptr if "masked" in self.arrays else None
+        return r
+
+
+@dataclass
+class DeviceHits:
+    n: int
+    arrays: dict
+
+    @classmethod
+    def alloc(cls, n: int, any_hit: bool = False) -> "DeviceHits":
+        arrs = {"prim": DeviceArray(4 * n), "t": DeviceArray(4 * n), "u": DeviceArray(4 * n),
+                "v": DeviceArray(4 * n)}
+        if any_hit:
+            arrs["occluded"] = DeviceArray(n)
+        return cls(n, arrs)
+
+    def struct(self) -> L.Hits:
+        h = L.Hits()
+        for k in ("prim", "t", "u", "v", "occluded"):
+            setattr(h, k, self.arrays[k].ptr if k in self.arrays else None)
+        return h
+
+    def download(self) -> dict:
+        out = {"prim": self.arrays["prim"].download(np.int32, self.n),
+               "t": self.arrays["t"].download(np.float32, self.n),
+               "u": self.arrays["u"].download(np.float32, self.n),
+               "v": self.arrays["v"].download(np.float32, self.n)}
+        if "occluded" in self.arrays:
+            out["occluded"] = self.arrays["occluded"].download(np.uint8, self.n)
+        return out
+
+
+def trace_opts(packet_size: int = 1, stack_lds: int = 0, blocks_per_cu: int = 0,
+               counters: DeviceArray | None = None) -> L.TraceOpts:
+    o = L.TraceOpts()
+    o.packet_size = packet_size
+    o.stack_lds = stack_lds
+    o.blocks_per_cu = blocks_per_cu
+    o.counters = counters.ptr if counters is not None else None
+    return o
+
+
+class Scene:
+    """A mesh plus its DynBVH on the device (one mh_scene)."""
+
+    def __init__(self, verts: np.ndarray, idx: np.ndarray):
+        lib = L.load()
+        self.verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        h = C.c_void_p()
+        L.check(lib.mh_scene_create(self.verts.ctypes.data, len(self.verts), self.idx.ctypes.data,
+                                    len(self.idx), C.byref(h)))
+        self.h = h.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.load().mh_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # AccelerationStructure::rebuild
+    def build(self, prim_kind: int = L.MH_PRIM_KENSLER_SHIRLEY, rtsah: bool = True,
+              threads: int = 0) -> "Scene":
+        o = L.BuildOpts(prim_kind, int(rtsah), threads)
+        L.check(L.load().mh_scene_build(self.h, C.byref(o)))
+        return self
+
+    # AccelerationStructure::buildFromFile (from memory)
+    def load_dynbvh(self, ids: np.ndarray, nodes: np.ndarray,
+                    prim_kind: int = L.MH_PRIM_KENSLER_SHIRLEY) -> "Scene":
+        ids = np.ascontiguousarray(ids, np.int32)
+        nodes = np.ascontiguousarray(nodes).view(np.uint8)
+        assert nodes.nbytes % 32 == 0
+        L.check(L.load().mh_scene_load_dynbvh(self.h, ids.ctypes.data, len(ids), nodes.ctypes.data,
+                                              nodes.nbytes // 32, prim_kind))
+        return self
+
+    def load_dynbvh_file(self, path: str, prim_kind: int = 0) -> "Scene":
+        L.check(L.load().mh_scene_load_dynbvh_file(self.h, str(path).encode(), prim_kind))
+        return self
+
+    def save_dynbvh_file(self, path: str) -> None:
+        L.check(L.load().mh_scene_save_dynbvh_file(self.h, str(path).encode()))
+
+    def export_dynbvh(self) -> tuple[np.ndarray, np.ndarray]:
+        lib = L.load()
+        ni, nn = C.c_uint32(), C.c_uint32()
+        L.check(lib.mh_scene_export_dynbvh(self.h, None, C.byref(ni), None, C.byref(nn)))
+        ids = np.empty(ni.value, np.int32)
+        nodes = np.empty(nn.value, L.NODE_DTYPE)
+        L.check(lib.mh_scene_export_dynbvh(self.h, ids.ctypes.data, C.byref(ni), nodes.ctypes.data,
+                                           C.byref(nn)))
+        return ids, nodes
+
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        lo = np.empty(3, np.float32)
+        hi = np.empty(3, np.float32)
+        L.check(L.load().mh_scene_bounds(self.h, lo.ctypes.data, hi.ctypes.data))
+        return lo, hi
+
+    def stats(self) -> dict:
+        nn, ni, d, b = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_size_t()
+        L.check(L.load().mh_scene_stats(self.h, C.byref(nn), C.byref(ni), C.byref(d), C.byref(b)))
+        return {"nodes": nn.value, "inner": ni.value, "depth": d.value, "device_bytes": b.value}
+
+    # AccelerationStructure::update
+    def set_vertices(self, verts: np.ndarray, stream=None) -> None:
+        v = np.ascontiguousarray(verts, np.float32)
+        assert v.size == self.verts.size
+        L.check(L.load().mh_scene_set_vertices(self.h, v.ctypes.data, 0, stream))
+        self.verts = v.reshape(-1, 3)
+
+    def refit(self, stream=None) -> None:
+        L.check(L.load().mh_scene_refit(self.h, stream))
+
+    def displace_sine(self, amp: float, freq: float, phase: float, stream=None) -> None:
+        L.check(L.load().mh_scene_displace_sine(self.h, amp, freq, phase, stream))
+
+    def device_vertices(self) -> np.ndarray:
+        p = C.c_void_p()
+        L.check(L.load().mh_scene_vertices_device(self.h, C.byref(p)))
+        out = np.empty(self.verts.size, np.float32)
+        L.check(L.load().mh_memcpy_d2h(out.ctypes.data, p.value, out.nbytes))
+        return out.reshape(-1, 3)
+
+    # Object::intersect
+    def trace(self, rays: DeviceRays, hits: DeviceHits, any_hit: bool = False,
+              opts: L.TraceOpts | None = None, stream=None) -> None:
+        r, h = rays.struct(), hits.struct()
+        L.check(L.load().mh_trace(self.h, C.byref(r), C.byref(h), rays.n,
+                                  L.MH_ANY_HIT if any_hit else L.MH_CLOSEST_HIT,
+                                  C.byref(opts) if opts is not None else None, stream))
+
+    def trace_host(self, rays: dict, any_hit: bool = False, packet_size: int = 1,
+                   stack_lds: int = 0) -> dict:
+        n = len(rays["ox"])
+        arr = {k: np.ascontiguousarray(rays[k], np.float32) for k in RAY_FIELDS}
+        m = rays.get("masked")
+        m = np.ascontiguousarray(m, np.uint8) if m is not None else None
+        out = {"prim": np.empty(n, np.int32), "t": np.empty(n, np.float32),
+               "u": np.empty(n, np.float32), "v": np.empty(n, np.float32)}
+        if any_hit:
+            out["occluded"] = np.empty(n, np.uint8)
+        r = L.Rays(*[arr[k].ctypes.data for k in RAY_FIELDS], L.ptr(m))
+        h = L.Hits(out["prim"].ctypes.data, out["t"].ctypes.data, out["u"].ctypes.data,
+                   out["v"].ctypes.data, out["occluded"].ctypes.data if any_hit else None)
+        o = trace_opts(packet_size, stack_lds)
+        L.check(L.load().mh_trace_host(self.h, C.byref(r), C.byref(h), n,
+                                       L.MH_ANY_HIT if any_hit else L.MH_CLOSEST_HIT, C.byref(o)))
+        return out
+
+
+def build_dynbvh_host(verts: np.ndarray, idx: np.ndarray, rtsah: bool = True,
+                      threads: int = 0) -> tuple[np.ndarray, np.ndarray, int]:
+    """Host SAH build (no device): (nodes[2T], ids[T], records in use)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    T = len(i)
+    ids = np.empty(T, np.int32)
+    nodes = np.empty(2 * T, L.NODE_DTYPE)
+    used = C.c_uint32()
+    o = L.BuildOpts(0, int(rtsah), threads)
+    L.check(L.load().mh_dynbvh_build_host(v.ctypes.data, len(v), i.ctypes.data, T, C.byref(o),
+                                          ids.ctypes.data, nodes.ctypes.data, C.byref(used)))
+    return nodes, ids, used.value
+
+
+def sync(stream=None) -> None:
+    L.check(L.load().mh_stream_sync(stream))
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    hbm = C.c_size_t()
+    L.check(L.load().mh_device_info(name, 256, C.byref(cu), C.byref(hbm)))
+    return {"arch": name.value.decode(), "cu_count": cu.value, "hbm_bytes": hbm.value}

@@ -1,0 +1,718 @@
+// C ABI of libmh.so (declared in include/mh.h): scene ownership, device upload, tree
+// import/export, refit and trace dispatch. No C++ exception leaves this file; every
+// entry point returns a status code and records a thread-local message.
+
+#include "mh.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mh_device.h"
+#include "sah_builder.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_OK(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return fail(MH_E_HIP, "%s: %s", #call, hipGetErrorString(e_));    \
+  } while (0)
+
+struct Lib {
+  std::mutex mu;
+  bool ready = false;
+  int device = 0;
+  int cu_count = 0;
+  size_t hbm = 0;
+  char name[256] = {0};
+};
+Lib g_lib;
+
+int ensure_init() {
+  std::lock_guard<std::mutex> lk(g_lib.mu);
+  if (g_lib.ready) {
+    HIP_OK(hipSetDevice(g_lib.device));
+    return MH_OK;
+  }
+  int count = 0;
+  HIP_OK(hipGetDeviceCount(&count));
+  if (g_lib.device < 0 || g_lib.device >= count)
+    return fail(MH_E_INVALID, "device %d out of range (%d devices)", g_lib.device, count);
+  HIP_OK(hipSetDevice(g_lib.device));
+  hipDeviceProp_t p;
+  HIP_OK(hipGetDeviceProperties(&p, g_lib.device));
+  g_lib.cu_count = p.multiProcessorCount;
+  g_lib.hbm = p.totalGlobalMem;
+  std::snprintf(g_lib.name, sizeof(g_lib.name), "%s", p.gcnArchName);
+  g_lib.ready = true;
+  return MH_OK;
+}
+
+// One device allocation, released with the owner.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int reserve(size_t n) {
+    if (n <= bytes && p) return MH_OK;
+    release();
+    if (n == 0) return MH_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(MH_E_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return MH_OK;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+}  // namespace
+
+struct mh_scene {
+  std::vector<float> verts;
+  std::vector<uint32_t> idx;
+  uint32_t nv = 0, nt = 0;
+  mh::DynBvh bvh;
+  bool built = false;
+  int prim_kind = MH_PRIM_KENSLER_SHIRLEY;
+  int root_ref = 0;
+  uint32_t n_inner = 0, n_leaves = 0, depth = 0;
+
+  DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, parent, slot, leaves, arrive;
+
+  std::mutex spill_mu;  // guards the per-stream spill areas
+  std::mutex stage_mu;  // serialises mh_trace_host calls on the staging buffer
+  std::map<void*, DevBuf*> spill;
+  DevBuf stage;
+  ~mh_scene() {
+    for (auto& kv : spill) delete kv.second;
+  }
+  size_t device_bytes() const {
+    return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
+           d_base.bytes + d_idx.bytes + parent.bytes + slot.bytes + leaves.bytes + arrive.bytes;
+  }
+};
+
+namespace {
+
+// Lays the reference tree out for the device: PairNodes in depth-first preorder of inner
+// records, leaf references, refit links and stack-depth bound, then uploads.
+int upload(mh_scene* s) {
+  const std::vector<mh::RefNode>& nodes = s->bvh.nodes;
+  const uint32_t nrec = (uint32_t)nodes.size();
+  const uint32_t T = s->nt;
+  std::vector<int32_t> pair_of(nrec, -1), parent(nrec, -1), slot(nrec, -1);
+  std::vector<int32_t> leaves;
+  std::vector<int32_t> big(T, 0);
+  std::vector<int32_t> order;  // inner records, preorder
+  struct It {
+    int32_t rec;
+    uint32_t depth;
+  };
+  std::vector<It> st;
+  st.push_back({0, 0});
+  uint32_t max_depth = 0;
+  while (!st.empty()) {
+    It it = st.back();
+    st.pop_back();
+    const mh::RefNode& n = nodes[it.rec];
+    max_depth = std::max(max_depth, it.depth);
+    if (n.leaf()) {
+      if (n.children < 0) return fail(MH_E_INVALID, "record %d: negative leaf size", it.rec);
+      if ((uint32_t)n.child + (uint32_t)n.children > T || n.child < 0)
+        return fail(MH_E_INVALID, "record %d: leaf range outside object_ids", it.rec);
+      leaves.push_back(it.rec);
+      if (n.children >= mh::kLeafCountInline) big[n.child] = n.children;
+      continue;
+    }
+    if (n.child < 1 || (uint32_t)n.child + 1 >= nrec)
+      return fail(MH_E_INVALID, "record %d: child %d outside the node array", it.rec, n.child);
+    if (n.axis > 2) return fail(MH_E_INVALID, "record %d: axis %d", it.rec, (int)n.axis);
+    pair_of[it.rec] = (int32_t)order.size();
+    order.push_back(it.rec);
+    if (order.size() > nrec) return fail(MH_E_INVALID, "node graph is not a tree");
+    parent[n.child] = it.rec;
+    parent[n.child + 1] = it.rec;
+    st.push_back({n.child + 1, it.depth + 1});
+    st.push_back({n.child, it.depth + 1});
+  }
+  auto encode = [&](int32_t rec) -> int32_t {
+    const mh::RefNode& n = nodes[rec];
+    if (!n.leaf()) return pair_of[rec];
+    const int c = std::min<int>(n.children, mh::kLeafCountInline);
+    return ~((n.child << mh::kLeafCountBits) | c);
+  };
+  std::vector<mh::PairNode> pairs(order.size());
+  for (size_t pi = 0; pi < order.size(); ++pi) {
+    const mh::RefNode& n = nodes[order[pi]];
+    mh::PairNode& P = pairs[pi];
+    for (int side = 0; side < 2; ++side) {
+      const mh::RefNode& c = nodes[n.child + side];
+      float* b = P.b + 6 * side;
+      b[0] = c.box.lo[0];
+      b[1] = c.box.hi[0];
+      b[2] = c.box.lo[1];
+      b[3] = c.box.hi[1];
+      b[4] = c.box.lo[2];
+      b[5] = c.box.hi[2];
+      P.ref[side] = encode(n.child + side);
+      slot[n.child + side] = (int32_t)(pi * 2 + side);
+    }
+    P.meta = (int32_t)n.axis | ((n.bits & 1) << 2);
+    P.node = order[pi];
+  }
+  s->root_ref = encode(0);
+  s->n_inner = (uint32_t)order.size();
+  s->n_leaves = (uint32_t)leaves.size();
+  s->depth = max_depth;
+
+  int rc;
+  if ((rc = s->refs.reserve(nrec * sizeof(mh::RefNode)))) return rc;
+  if ((rc = s->pairs.reserve(std::max<size_t>(1, pairs.size()) * sizeof(mh::PairNode)))) return rc;
+  if ((rc = s->tris.reserve((size_t)T * 12 * sizeof(float)))) return rc;
+  if ((rc = s->ids.reserve((size_t)T * sizeof(int32_t)))) return rc;
+  if ((rc = s->big.reserve((size_t)T * sizeof(int32_t)))) return rc;
+  if ((rc = s->parent.reserve(nrec * sizeof(int32_t)))) return rc;
+  if ((rc = s->slot.reserve(nrec * sizeof(int32_t)))) return rc;
+  if ((rc = s->leaves.reserve(std::max<size_t>(1, leaves.size()) * sizeof(int32_t)))) return rc;
+  if ((rc = s->arrive.reserve(nrec * sizeof(uint32_t)))) return rc;
+  HIP_OK(hipMemcpy(s->refs.p, nodes.data(), nrec * sizeof(mh::RefNode), hipMemcpyHostToDevice));
+  if (!pairs.empty())
+    HIP_OK(hipMemcpy(s->pairs.p, pairs.data(), pairs.size() * sizeof(mh::PairNode),
+                     hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->ids.p, s->bvh.ids.data(), (size_t)T * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->big.p, big.data(), (size_t)T * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->parent.p, parent.data(), nrec * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->slot.p, slot.data(), nrec * 4, hipMemcpyHostToDevice));
+  if (!leaves.empty())
+    HIP_OK(hipMemcpy(s->leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(s->arrive.p, 0, nrec * 4));
+
+  mh::DevRefit r{};
+  r.tris = s->tris.as<float>();
+  r.obj_ids = s->ids.as<int32_t>();
+  r.verts = s->d_verts.as<float>();
+  r.idx = s->d_idx.as<uint32_t>();
+  r.n_slots = (int32_t)T;
+  r.prim_kind = s->prim_kind;
+  HIP_OK((hipError_t)mh::launch_prim_setup(r, nullptr));
+  HIP_OK(hipDeviceSynchronize());
+  s->built = true;
+  return MH_OK;
+}
+
+int validate_ids(const std::vector<int32_t>& ids, uint32_t T) {
+  if (ids.size() != T) return fail(MH_E_INVALID, "object_ids has %zu entries for %u triangles",
+                                   ids.size(), T);
+  std::vector<uint8_t> seen(T, 0);
+  for (int32_t id : ids) {
+    if (id < 0 || (uint32_t)id >= T || seen[id])
+      return fail(MH_E_INVALID, "object_ids is not a permutation of the triangles");
+    seen[id] = 1;
+  }
+  return MH_OK;
+}
+
+mh::DevScene dev_scene(const mh_scene* s) {
+  mh::DevScene d;
+  d.pairs = s->pairs.as<const mh::PairNode>();
+  d.refs = s->refs.as<const mh::RefNode>();
+  d.tris = s->tris.as<const float>();
+  d.obj_ids = s->ids.as<const int32_t>();
+  d.big_count = s->big.as<const int32_t>();
+  d.root_ref = s->root_ref;
+  d.prim_kind = s->prim_kind;
+  return d;
+}
+
+int choose_stack(const mh_trace_opts* o) {
+  const int v = o ? o->stack_lds : 0;
+  return (v == 8 || v == 16 || v == 32) ? v : 16;
+}
+
+int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int64_t n,
+               uint32_t flags, const mh_trace_opts* opts, void* stream) {
+  mh_scene* s = const_cast<mh_scene*>(cs);
+  if (!s || !rays || !hits) return fail(MH_E_INVALID, "null scene, rays or hits");
+  if (!s->built) return fail(MH_E_STATE, "scene has no tree: call mh_scene_build first");
+  if (n < 0) return fail(MH_E_INVALID, "negative ray count");
+  if (flags & ~MH_ANY_HIT) return fail(MH_E_INVALID, "unknown trace flags 0x%x", flags);
+  if (n == 0) return MH_OK;
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz || !rays->tmax)
+    return fail(MH_E_INVALID, "ray arrays must be non-null");
+  const bool any = flags & MH_ANY_HIT;
+  if (any ? !hits->occluded : !hits->prim)
+    return fail(MH_E_INVALID, any ? "any-hit needs hits.occluded" : "closest-hit needs hits.prim");
+  const int psize = opts && opts->packet_size ? opts->packet_size : 1;
+  if (psize < 1 || psize > 64) return fail(MH_E_INVALID, "packet_size %d not in 1..64", psize);
+  int rc;
+  if ((rc = ensure_init())) return rc;
+
+  mh::LaunchCfg cfg{};
+  cfg.any_hit = any;
+  cfg.packet_size = psize;
+  cfg.counters = opts ? opts->counters : nullptr;
+  if (psize > 1) {
+    if (s->depth + 2 > 128)
+      return fail(MH_E_UNSUPPORTED, "tree depth %u exceeds the packet stack (126)", s->depth);
+  } else {
+    const int S = choose_stack(opts);
+    const int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : 4;
+    const int64_t want = (n + 255) / 256;
+    const int64_t cap = (int64_t)g_lib.cu_count * bpc;
+    cfg.blocks = (int)std::max<int64_t>(1, std::min(want, cap));
+    cfg.stack_lds = S;
+    cfg.spill_stride = (int64_t)cfg.blocks * 256;
+    const int64_t need = (int64_t)s->depth > S ? (int64_t)s->depth - S : 0;
+    if (need > 0) {
+      std::lock_guard<std::mutex> lk(s->spill_mu);
+      DevBuf*& b = s->spill[stream];
+      if (!b) b = new (std::nothrow) DevBuf();
+      if (!b) return fail(MH_E_NOMEM, "spill descriptor");
+      if ((rc = b->reserve((size_t)need * cfg.spill_stride * 8))) return rc;
+      cfg.spill = b->as<int32_t>();
+    }
+  }
+  mh::DevRays dr{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax,
+                 rays->masked};
+  mh::DevHits dh{hits->prim, hits->t, hits->u, hits->v, hits->occluded};
+  HIP_OK((hipError_t)mh::launch_trace(dev_scene(s), dr, dh, n, cfg, stream));
+  return MH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_abi_version(void) { return MH_ABI_VERSION; }
+
+const char* mh_last_error(void) { return g_err.c_str(); }
+
+int mh_init(int device) {
+  {
+    std::lock_guard<std::mutex> lk(g_lib.mu);
+    if (g_lib.ready && g_lib.device != device)
+      return fail(MH_E_STATE, "already initialised on device %d", g_lib.device);
+    g_lib.device = device;
+  }
+  return ensure_init();
+}
+
+int mh_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", g_lib.name);
+  if (cu_count) *cu_count = g_lib.cu_count;
+  if (hbm_bytes) *hbm_bytes = g_lib.hbm;
+  return MH_OK;
+}
+
+int mh_scene_create(const float* verts, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris,
+                    mh_scene** out) {
+  if (!out) return fail(MH_E_INVALID, "null output pointer");
+  *out = nullptr;
+  if (!verts || !idx || n_verts == 0 || n_tris == 0)
+    return fail(MH_E_INVALID, "empty mesh (%u vertices, %u triangles)", n_verts, n_tris);
+  if (n_tris >= mh::kMaxObjects)
+    return fail(MH_E_UNSUPPORTED, "%u triangles exceeds the limit %u", n_tris, mh::kMaxObjects);
+  for (size_t i = 0; i < 3 * (size_t)n_tris; ++i)
+    if (idx[i] >= n_verts) return fail(MH_E_INVALID, "index %zu = %u out of range", i, idx[i]);
+  int rc = ensure_init();
+  if (rc) return rc;
+  mh_scene* s = new (std::nothrow) mh_scene();
+  if (!s) return fail(MH_E_NOMEM, "scene");
+  try {
+    s->verts.assign(verts, verts + 3 * (size_t)n_verts);
+    s->idx.assign(idx, idx + 3 * (size_t)n_tris);
+  } catch (...) {
+    delete s;
+    return fail(MH_E_NOMEM, "host copy of the mesh");
+  }
+  s->nv = n_verts;
+  s->nt = n_tris;
+  const size_t vb = 3 * (size_t)n_verts * 4, ib = 3 * (size_t)n_tris * 4;
+  if ((rc = s->d_verts.reserve(vb)) || (rc = s->d_base.reserve(vb)) ||
+      (rc = s->d_idx.reserve(ib))) {
+    delete s;
+    return rc;
+  }
+  hipError_t e = hipMemcpy(s->d_verts.p, verts, vb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->d_base.p, verts, vb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->d_idx.p, idx, ib, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(MH_E_HIP, "mesh upload: %s", hipGetErrorString(e));
+  }
+  *out = s;
+  return MH_OK;
+}
+
+int mh_scene_destroy(mh_scene* scene) {
+  if (!scene) return MH_OK;
+  (void)ensure_init();
+  (void)hipDeviceSynchronize();
+  delete scene;
+  return MH_OK;
+}
+
+int mh_scene_build(mh_scene* s, const mh_build_opts* opts) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  const int kind = opts ? opts->prim_kind : MH_PRIM_KENSLER_SHIRLEY;
+  if (kind != MH_PRIM_KENSLER_SHIRLEY && kind != MH_PRIM_WALD)
+    return fail(MH_E_INVALID, "unknown prim_kind %d", kind);
+  const int rtsah = opts ? opts->rtsah : 1;
+  const int threads = opts ? opts->threads : 0;
+  int rc = ensure_init();
+  if (rc) return rc;
+  try {
+    s->bvh = mh::build_dynbvh(s->verts.data(), s->nv, s->idx.data(), s->nt, threads);
+    if (rtsah) mh::compute_rtsah(s->bvh.nodes, threads);
+  } catch (const std::bad_alloc&) {
+    return fail(MH_E_NOMEM, "host memory for the SAH build");
+  } catch (const std::exception& ex) {
+    return fail(MH_E_INVALID, "build: %s", ex.what());
+  }
+  s->prim_kind = kind;
+  return upload(s);
+}
+
+int mh_dynbvh_build_host(const float* verts, uint32_t n_verts, const uint32_t* idx,
+                         uint32_t n_tris, const mh_build_opts* opts, int32_t* ids, void* nodes,
+                         uint32_t* num_nodes) {
+  if (!verts || !idx || !ids || !nodes) return fail(MH_E_INVALID, "null argument");
+  if (n_verts == 0 || n_tris == 0) return fail(MH_E_INVALID, "empty mesh");
+  if (n_tris >= mh::kMaxObjects)
+    return fail(MH_E_UNSUPPORTED, "%u triangles exceeds the limit %u", n_tris, mh::kMaxObjects);
+  for (size_t i = 0; i < 3 * (size_t)n_tris; ++i)
+    if (idx[i] >= n_verts) return fail(MH_E_INVALID, "index %zu = %u out of range", i, idx[i]);
+  const int rtsah = opts ? opts->rtsah : 1;
+  const int threads = opts ? opts->threads : 0;
+  try {
+    mh::DynBvh b = mh::build_dynbvh(verts, n_verts, idx, n_tris, threads);
+    if (rtsah) mh::compute_rtsah(b.nodes, threads);
+    std::memcpy(ids, b.ids.data(), b.ids.size() * 4);
+    std::memcpy(nodes, b.nodes.data(), b.nodes.size() * sizeof(mh::RefNode));
+    if (num_nodes) *num_nodes = (uint32_t)b.num_nodes;
+  } catch (const std::bad_alloc&) {
+    return fail(MH_E_NOMEM, "host memory for the SAH build");
+  } catch (const std::exception& ex) {
+    return fail(MH_E_INVALID, "build: %s", ex.what());
+  }
+  return MH_OK;
+}
+
+int mh_scene_load_dynbvh(mh_scene* s, const int32_t* ids, uint32_t n_ids, const void* nodes,
+                         uint32_t n_nodes, int prim_kind) {
+  if (!s || !ids || !nodes) return fail(MH_E_INVALID, "null argument");
+  if (prim_kind != MH_PRIM_KENSLER_SHIRLEY && prim_kind != MH_PRIM_WALD)
+    return fail(MH_E_INVALID, "unknown prim_kind %d", prim_kind);
+  if (n_nodes == 0) return fail(MH_E_INVALID, "empty node array");
+  int rc = ensure_init();
+  if (rc) return rc;
+  std::vector<int32_t> idv(ids, ids + n_ids);
+  if ((rc = validate_ids(idv, s->nt))) return rc;
+  s->bvh.ids = std::move(idv);
+  const mh::RefNode* nn = static_cast<const mh::RefNode*>(nodes);
+  s->bvh.nodes.assign(nn, nn + n_nodes);
+  s->bvh.num_nodes = 0;
+  s->prim_kind = prim_kind;
+  s->built = false;
+  return upload(s);
+}
+
+int mh_scene_load_dynbvh_file(mh_scene* s, const char* path, int prim_kind) {
+  if (!s || !path) return fail(MH_E_INVALID, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return fail(MH_E_IO, "cannot open %s", path);
+  uint32_t n_ids = 0, n_nodes = 0;
+  in.read(reinterpret_cast<char*>(&n_ids), 4);
+  if (!in || n_ids != s->nt) return fail(MH_E_IO, "%s: bad object_ids count", path);
+  std::vector<int32_t> ids(n_ids);
+  in.read(reinterpret_cast<char*>(ids.data()), (std::streamsize)n_ids * 4);
+  in.read(reinterpret_cast<char*>(&n_nodes), 4);
+  if (!in || n_nodes == 0 || n_nodes > 4u * n_ids + 4u) return fail(MH_E_IO, "%s: bad node count", path);
+  std::vector<mh::RefNode> nodes(n_nodes);
+  in.read(reinterpret_cast<char*>(nodes.data()), (std::streamsize)n_nodes * 32);
+  if (!in) return fail(MH_E_IO, "%s: truncated", path);
+  return mh_scene_load_dynbvh(s, ids.data(), n_ids, nodes.data(), n_nodes, prim_kind);
+}
+
+int mh_scene_export_dynbvh(const mh_scene* s, int32_t* ids, uint32_t* n_ids, void* nodes,
+                           uint32_t* n_nodes) {
+  if (!s || !n_ids || !n_nodes) return fail(MH_E_INVALID, "null argument");
+  if (!s->built) return fail(MH_E_STATE, "scene has no tree");
+  const uint32_t ni = (uint32_t)s->bvh.ids.size(), nn = (uint32_t)s->bvh.nodes.size();
+  if (!ids && !nodes) {
+    *n_ids = ni;
+    *n_nodes = nn;
+    return MH_OK;
+  }
+  if (*n_ids < ni || *n_nodes < nn) return fail(MH_E_INVALID, "export arrays too small");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (ids) std::memcpy(ids, s->bvh.ids.data(), (size_t)ni * 4);
+  if (nodes) HIP_OK(hipMemcpy(nodes, s->refs.p, (size_t)nn * 32, hipMemcpyDeviceToHost));
+  *n_ids = ni;
+  *n_nodes = nn;
+  return MH_OK;
+}
+
+int mh_scene_save_dynbvh_file(const mh_scene* s, const char* path) {
+  if (!s || !path) return fail(MH_E_INVALID, "null argument");
+  uint32_t ni = 0, nn = 0;
+  int rc = mh_scene_export_dynbvh(s, nullptr, &ni, nullptr, &nn);
+  if (rc) return rc;
+  std::vector<int32_t> ids(ni);
+  std::vector<mh::RefNode> nodes(nn);
+  if ((rc = mh_scene_export_dynbvh(s, ids.data(), &ni, nodes.data(), &nn))) return rc;
+  std::ofstream out(path, std::ios::binary);
+  if (!out) return fail(MH_E_IO, "cannot create %s", path);
+  out.write(reinterpret_cast<const char*>(&ni), 4);
+  out.write(reinterpret_cast<const char*>(ids.data()), (std::streamsize)ni * 4);
+  out.write(reinterpret_cast<const char*>(&nn), 4);
+  out.write(reinterpret_cast<const char*>(nodes.data()), (std::streamsize)nn * 32);
+  if (!out) return fail(MH_E_IO, "write to %s failed", path);
+  return MH_OK;
+}
+
+int mh_scene_bounds(const mh_scene* s, float lo[3], float hi[3]) {
+  if (!s || !lo || !hi) return fail(MH_E_INVALID, "null argument");
+  if (!s->built) return fail(MH_E_STATE, "scene has no tree");
+  int rc = ensure_init();
+  if (rc) return rc;
+  mh::RefNode root;
+  HIP_OK(hipMemcpy(&root, s->refs.p, sizeof(root), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = root.box.lo[k];
+    hi[k] = root.box.hi[k];
+  }
+  return MH_OK;
+}
+
+int mh_scene_stats(const mh_scene* s, uint32_t* n_nodes, uint32_t* n_inner, uint32_t* depth,
+                   size_t* device_bytes) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  if (!s->built) return fail(MH_E_STATE, "scene has no tree");
+  if (n_nodes) *n_nodes = s->n_inner + s->n_leaves;
+  if (n_inner) *n_inner = s->n_inner;
+  if (depth) *depth = s->depth;
+  if (device_bytes) *device_bytes = s->device_bytes();
+  return MH_OK;
+}
+
+int mh_scene_set_vertices(mh_scene* s, const float* verts, int on_device, void* stream) {
+  if (!s || !verts) return fail(MH_E_INVALID, "null argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  const size_t vb = 3 * (size_t)s->nv * 4;
+  HIP_OK(hipMemcpyAsync(s->d_verts.p, verts, vb,
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        reinterpret_cast<hipStream_t>(stream)));
+  if (!on_device) std::memcpy(s->verts.data(), verts, vb);
+  return MH_OK;
+}
+
+int mh_scene_vertices_device(mh_scene* s, float** verts) {
+  if (!s || !verts) return fail(MH_E_INVALID, "null argument");
+  *verts = s->d_verts.as<float>();
+  return MH_OK;
+}
+
+int mh_scene_displace_sine(mh_scene* s, float amp, float freq, float phase, void* stream) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIP_OK((hipError_t)mh::launch_displace_sine(s->d_verts.as<float>(), s->d_base.as<float>(),
+                                              s->nv, amp, freq, phase, stream));
+  return MH_OK;
+}
+
+int mh_scene_refit(mh_scene* s, void* stream) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  if (!s->built) return fail(MH_E_STATE, "scene has no tree");
+  int rc = ensure_init();
+  if (rc) return rc;
+  mh::DevRefit r{};
+  r.refs = s->refs.as<mh::RefNode>();
+  r.pairs = s->pairs.as<mh::PairNode>();
+  r.tris = s->tris.as<float>();
+  r.obj_ids = s->ids.as<int32_t>();
+  r.verts = s->d_verts.as<float>();
+  r.idx = s->d_idx.as<uint32_t>();
+  r.parent = s->parent.as<int32_t>();
+  r.slot = s->slot.as<int32_t>();
+  r.leaves = s->leaves.as<int32_t>();
+  r.arrive = s->arrive.as<uint32_t>();
+  r.n_leaves = (int32_t)s->n_leaves;
+  r.n_slots = (int32_t)s->nt;
+  r.prim_kind = s->prim_kind;
+  HIP_OK((hipError_t)mh::launch_refit(r, stream));
+  return MH_OK;
+}
+
+int mh_trace(const mh_scene* s, const mh_rays* rays, const mh_hits* hits, int64_t n,
+             uint32_t flags, const mh_trace_opts* opts, void* stream) {
+  return trace_impl(s, rays, hits, n, flags, opts, stream);
+}
+
+int mh_trace_host(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int64_t n,
+                  uint32_t flags, const mh_trace_opts* opts) {
+  mh_scene* s = const_cast<mh_scene*>(cs);
+  if (!s || !rays || !hits) return fail(MH_E_INVALID, "null argument");
+  if (n <= 0) return n == 0 ? MH_OK : fail(MH_E_INVALID, "negative ray count");
+  int rc = ensure_init();
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->stage_mu);
+  // Staging layout: 7 float arrays + mask bytes in, 4 result arrays + occluded bytes out.
+  const size_t f = (size_t)n * 4, al = 256;
+  auto up = [&](size_t x) { return (x + al - 1) / al * al; };
+  const size_t in_bytes = 7 * up(f) + up((size_t)n);
+  const size_t out_bytes = 4 * up(f) + up((size_t)n);
+  if ((rc = s->stage.reserve(in_bytes + out_bytes))) return rc;
+  char* base = s->stage.as<char>();
+  float* d[7];
+  const float* src[7] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+  for (int i = 0; i < 7; ++i) {
+    if (!src[i]) return fail(MH_E_INVALID, "ray array %d is null", i);
+    d[i] = reinterpret_cast<float*>(base + i * up(f));
+    HIP_OK(hipMemcpy(d[i], src[i], f, hipMemcpyHostToDevice));
+  }
+  uint8_t* dm = nullptr;
+  if (rays->masked) {
+    dm = reinterpret_cast<uint8_t*>(base + 7 * up(f));
+    HIP_OK(hipMemcpy(dm, rays->masked, (size_t)n, hipMemcpyHostToDevice));
+  }
+  char* ob = base + in_bytes;
+  mh_hits dh{};
+  dh.prim = hits->prim ? reinterpret_cast<int32_t*>(ob) : nullptr;
+  dh.t = hits->t ? reinterpret_cast<float*>(ob + up(f)) : nullptr;
+  dh.u = hits->u ? reinterpret_cast<float*>(ob + 2 * up(f)) : nullptr;
+  dh.v = hits->v ? reinterpret_cast<float*>(ob + 3 * up(f)) : nullptr;
+  dh.occluded = hits->occluded ? reinterpret_cast<uint8_t*>(ob + 4 * up(f)) : nullptr;
+  mh_rays dr{d[0], d[1], d[2], d[3], d[4], d[5], d[6], dm};
+  if ((rc = trace_impl(s, &dr, &dh, n, flags, opts, nullptr))) return rc;
+  HIP_OK(hipDeviceSynchronize());
+  if (hits->prim) HIP_OK(hipMemcpy(hits->prim, dh.prim, f, hipMemcpyDeviceToHost));
+  if (hits->t) HIP_OK(hipMemcpy(hits->t, dh.t, f, hipMemcpyDeviceToHost));
+  if (hits->u) HIP_OK(hipMemcpy(hits->u, dh.u, f, hipMemcpyDeviceToHost));
+  if (hits->v) HIP_OK(hipMemcpy(hits->v, dh.v, f, hipMemcpyDeviceToHost));
+  if (hits->occluded)
+    HIP_OK(hipMemcpy(hits->occluded, dh.occluded, (size_t)n, hipMemcpyDeviceToHost));
+  return MH_OK;
+}
+
+int mh_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(MH_E_INVALID, "null argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return fail(MH_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return MH_OK;
+}
+
+int mh_free(void* ptr) {
+  if (!ptr) return MH_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIP_OK(hipFree(ptr));
+  return MH_OK;
+}
+
+int mh_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return MH_OK;
+}
+
+int mh_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return MH_OK;
+}
+
+int mh_stream_create(void** stream) {
+  if (!stream) return fail(MH_E_INVALID, "null argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t s;
+  HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return MH_OK;
+}
+
+int mh_stream_destroy(void* stream) {
+  if (!stream) return MH_OK;
+  HIP_OK(hipStreamDestroy(reinterpret_cast<hipStream_t>(stream)));
+  return MH_OK;
+}
+
+int mh_stream_sync(void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIP_OK(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+  return MH_OK;
+}
+
+int mh_time_trace(const mh_scene* s, const mh_rays* rays, const mh_hits* hits, int64_t n,
+                  uint32_t flags, const mh_trace_opts* opts, void* stream, int iters, float* ms) {
+  if (!ms || iters < 1) return fail(MH_E_INVALID, "bad timing arguments");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  HIP_OK(hipEventRecord(a, st));
+  for (int i = 0; i < iters; ++i)
+    if ((rc = trace_impl(s, rays, hits, n, flags, opts, stream))) {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      return rc;
+    }
+  HIP_OK(hipEventRecord(b, st));
+  HIP_OK(hipEventSynchronize(b));
+  HIP_OK(hipEventElapsedTime(ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return MH_OK;
+}
+
+}  // extern "C"

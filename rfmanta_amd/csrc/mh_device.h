@@ -1,0 +1,94 @@
+// Device data layout shared by the HIP kernels and the host library.
+//
+// HBM layout of one scene (all arrays 256-byte aligned, one allocation each):
+//   refs      RefNode[2T]      reference DynBVH records, 32 B (packet traversal, refit)
+//   pairs     PairNode[I]      one 64-B record per inner node holding both child boxes
+//                              (single-ray traversal; one 64-B fetch decides both children)
+//   tris      float4[3T]       per object slot (object_ids order), 48 B:
+//                              Kensler-Shirley: p0, e0 = p1-p0, e1 = p0-p2, n = e0 x e1
+//                              Wald: n_u n_v n_d k | b_nu b_nv b_d n_k | c_nu c_nv c_d 0
+//   obj_ids   int32[T]         object_ids
+//   big_count int32[T]         leaf size at a leaf's first slot when it is >= 15
+//   verts     float[3V], idx uint32[3T], base_verts float[3V]    mesh (refit, animation)
+//   parent    int32[2T], slot int32[2T], leaves int32[L], arrive uint32[2T]   refit links
+#pragma once
+
+#include <cstdint>
+
+#include "sah_builder.h"
+
+namespace mh {
+
+// Child reference inside a PairNode: >= 0 is an inner-node (pair) index; < 0 is a leaf,
+// ~((first_slot << 4) | c) with c = min(count, 15); c == 15 means "read big_count".
+constexpr int kLeafCountBits = 4;
+constexpr int kLeafCountInline = 15;
+constexpr uint32_t kMaxObjects = 1u << 27;
+
+struct alignas(16) PairNode {
+  // child 0: lo.x hi.x lo.y hi.y | lo.z hi.z ; child 1: lo.x hi.x | lo.y hi.y lo.z hi.z
+  float b[12];
+  int32_t ref[2];
+  int32_t meta;  // bits 0-1 split axis, bit 2 isLeftCheaper
+  int32_t node;  // reference record id of this inner node
+};
+static_assert(sizeof(PairNode) == 64, "pair node is 64 bytes");
+
+struct DevScene {
+  const PairNode* pairs;
+  const RefNode* refs;
+  const float* tris;  // 12 floats per slot
+  const int32_t* obj_ids;
+  const int32_t* big_count;
+  int32_t root_ref;   // pair-encoded reference of record 0
+  int32_t prim_kind;  // 0 KS, 1 Wald
+};
+
+struct DevRays {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+  const uint8_t* masked;
+};
+
+struct DevHits {
+  int32_t* prim;
+  float *t, *u, *v;
+  uint8_t* occluded;
+};
+
+struct LaunchCfg {
+  int any_hit;
+  int packet_size;    // 1 = single-ray kernel
+  int stack_lds;      // 8/16/32 (single-ray)
+  int blocks;         // grid size in blocks of 256
+  int64_t* counters;  // device int64[2] or null
+  int32_t* spill;     // per-thread stack spill (single-ray)
+  int64_t spill_stride;
+};
+
+// Launchers (trace.hip). Return hipError_t as int.
+int launch_trace(const DevScene& sc, const DevRays& rays, const DevHits& hits, int64_t n,
+                 const LaunchCfg& cfg, void* stream);
+
+struct DevRefit {
+  RefNode* refs;
+  PairNode* pairs;
+  float* tris;
+  const int32_t* obj_ids;
+  const float* verts;
+  const uint32_t* idx;
+  const int32_t* parent;  // parent record id, -1 for the root
+  const int32_t* slot;    // pair index * 2 + side for a non-root record, -1 for the root
+  const int32_t* leaves;  // leaf record ids
+  uint32_t* arrive;       // per-record arrival counters (never reset: odd = second arrival)
+  int32_t n_leaves;
+  int32_t n_slots;        // T
+  int32_t prim_kind;
+};
+
+// refit.hip
+int launch_prim_setup(const DevRefit& r, void* stream);  // tris from verts (all slots)
+int launch_refit(const DevRefit& r, void* stream);       // tris + bounds, bottom-up
+int launch_displace_sine(float* verts, const float* base, int64_t n_verts, float amp, float freq,
+                         float phase, void* stream);
+
+}  // namespace mh

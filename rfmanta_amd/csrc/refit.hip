@@ -1,0 +1,232 @@
+// Per-frame refit kernels for gfx950.
+//
+// k_refit: one lane per leaf record. The lane rebuilds the triangle records of its leaf
+// from the current vertices, forms the leaf box (Mesh::getBBox + BBox::extendByBox,
+// Model/Groups/Mesh.cc:228-237, DynBVH.cc:2091-2097), stores it in the reference record
+// and in the parent's PairNode, then climbs: at each parent an agent-scope acq_rel
+// counter lets the second child to arrive form the union of both child boxes
+// (DynBVH.cc:2098-2117, updateBounds<false> without rotations) and keep climbing. The
+// counters are never cleared: each refit adds exactly two arrivals per inner record, so
+// an odd previous value identifies the second arrival.
+//
+// Box arithmetic is exact (min/max of the same fp32 values), so the result is
+// bit-identical to a serial recomputation in any arrival order.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "mh_device.h"
+
+namespace mh {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float pick_lo(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float pick_hi(float a, float b) { return a > b ? a : b; }
+
+struct Tri {
+  float p[3][3];
+};
+
+__device__ __forceinline__ Tri load_tri(const float* verts, const uint32_t* idx, int tri) {
+  Tri t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t vi = idx[3 * (size_t)tri + k];
+    t.p[k][0] = verts[3 * (size_t)vi + 0];
+    t.p[k][1] = verts[3 * (size_t)vi + 1];
+    t.p[k][2] = verts[3 * (size_t)vi + 2];
+  }
+  return t;
+}
+
+// Kensler-Shirley record: p0, e0 = p1 - p0, e1 = p0 - p2, n = Cross(e0, e1)
+// (KenslerShirleyTriangle.cc:341-345).
+__device__ __forceinline__ void ks_record(const Tri& t, float* o) {
+  const float e0x = t.p[1][0] - t.p[0][0], e0y = t.p[1][1] - t.p[0][1],
+              e0z = t.p[1][2] - t.p[0][2];
+  const float e1x = t.p[0][0] - t.p[2][0], e1y = t.p[0][1] - t.p[2][1],
+              e1z = t.p[0][2] - t.p[2][2];
+  o[0] = t.p[0][0];
+  o[1] = t.p[0][1];
+  o[2] = t.p[0][2];
+  o[3] = e0x;
+  o[4] = e0y;
+  o[5] = e0z;
+  o[6] = e1x;
+  o[7] = e1y;
+  o[8] = e1z;
+  o[9] = e0y * e1z - e0z * e1y;
+  o[10] = e0z * e1x - e0x * e1z;
+  o[11] = e0x * e1y - e0y * e1x;
+}
+
+// WaldTriangle::setPoints(v0, v1, v2) (WaldTriangle.cc:64-127).
+__device__ __forceinline__ void wald_record(const Tri& t, float* o) {
+  const float* p1 = t.p[0];
+  const float* p2 = t.p[1];
+  const float* p3 = t.p[2];
+  const float ax = p2[0] - p1[0], ay = p2[1] - p1[1], az = p2[2] - p1[2];
+  const float bx = p3[0] - p1[0], by = p3[1] - p1[1], bz = p3[2] - p1[2];
+  float nrm[3] = {ay * bz - az * by, az * bx - ax * bz, ax * by - ay * bx};
+  const float l = sqrtf(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  const float scale = 1 / l;
+  nrm[0] *= scale;
+  nrm[1] *= scale;
+  nrm[2] *= scale;
+  int n = 0;
+  if (fabsf(nrm[1]) > fabsf(nrm[n])) n = 1;
+  if (fabsf(nrm[2]) > fabsf(nrm[n])) n = 2;
+  const int u = n == 0 ? 1 : (n == 1 ? 2 : 0);
+  const int v = n == 0 ? 2 : (n == 1 ? 0 : 1);
+  const float inv_nk = 1 / nrm[n];
+  const float n_u = nrm[u] * inv_nk;
+  const float n_v = nrm[v] * inv_nk;
+  const float n_d = p1[u] * n_u + p1[v] * n_v + p1[n];
+  float c_nu = +p2[v] - p1[v];
+  float c_nv = -p2[u] + p1[u];
+  float c_d = -(p1[u] * c_nu + p1[v] * c_nv);
+  float s = 1.f / (p3[u] * c_nu + p3[v] * c_nv + c_d);
+  c_nu *= s;
+  c_nv *= s;
+  c_d *= s;
+  float b_nu = +p3[v] - p1[v];
+  float b_nv = -p3[u] + p1[u];
+  float b_d = -(p1[u] * b_nu + p1[v] * b_nv);
+  s = 1.f / (p2[u] * b_nu + p2[v] * b_nv + b_d);
+  b_nu *= s;
+  b_nv *= s;
+  b_d *= s;
+  o[0] = n_u;
+  o[1] = n_v;
+  o[2] = n_d;
+  o[3] = __int_as_float(n);
+  o[4] = b_nu;
+  o[5] = b_nv;
+  o[6] = b_d;
+  o[7] = nrm[n];
+  o[8] = c_nu;
+  o[9] = c_nv;
+  o[10] = c_d;
+  o[11] = 0.f;
+}
+
+__device__ __forceinline__ void write_record(const DevRefit& r, int slot, const Tri& t) {
+  float o[12];
+  if (r.prim_kind == 1)
+    wald_record(t, o);
+  else
+    ks_record(t, o);
+  float4* dst = reinterpret_cast<float4*>(r.tris + 12 * (size_t)slot);
+  dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+  dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+  dst[2] = make_float4(o[8], o[9], o[10], o[11]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_prim_setup(DevRefit r) {
+  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= r.n_slots) return;
+  write_record(r, (int)s, load_tri(r.verts, r.idx, r.obj_ids[s]));
+}
+
+__device__ __forceinline__ void store_box(const DevRefit& r, int rec, const float lo[3],
+                                          const float hi[3]) {
+  RefNode* nd = r.refs + rec;
+  nd->box.lo[0] = lo[0];
+  nd->box.lo[1] = lo[1];
+  nd->box.lo[2] = lo[2];
+  nd->box.hi[0] = hi[0];
+  nd->box.hi[1] = hi[1];
+  nd->box.hi[2] = hi[2];
+  const int code = r.slot[rec];
+  if (code >= 0) {
+    float* b = r.pairs[code >> 1].b + 6 * (code & 1);
+    b[0] = lo[0];
+    b[1] = hi[0];
+    b[2] = lo[1];
+    b[3] = hi[1];
+    b[4] = lo[2];
+    b[5] = hi[2];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit(DevRefit r) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= r.n_leaves) return;
+  int rec = r.leaves[i];
+  const RefNode leaf = r.refs[rec];
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int k = 0; k < leaf.children; ++k) {
+    const int slot = leaf.child + k;
+    const Tri t = load_tri(r.verts, r.idx, r.obj_ids[slot]);
+    write_record(r, slot, t);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float tlo = pick_lo(pick_lo(t.p[0][a], t.p[1][a]), t.p[2][a]);
+      const float thi = pick_hi(pick_hi(t.p[0][a], t.p[1][a]), t.p[2][a]);
+      lo[a] = pick_lo(lo[a], tlo);
+      hi[a] = pick_hi(hi[a], thi);
+    }
+  }
+  store_box(r, rec, lo, hi);
+  for (;;) {
+    const int p = r.parent[rec];
+    if (p < 0) break;
+    const unsigned old =
+        __hip_atomic_fetch_add(r.arrive + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if ((old & 1u) == 0u) break;  // the sibling subtree is still being refit
+    const int c = r.refs[p].child;
+    const RefNode a = r.refs[c];
+    const RefNode b = r.refs[c + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = pick_lo(pick_lo(FLT_MAX, a.box.lo[k]), b.box.lo[k]);
+      hi[k] = pick_hi(pick_hi(-FLT_MAX, a.box.hi[k]), b.box.hi[k]);
+    }
+    store_box(r, p, lo, hi);
+    rec = p;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_displace_sine(float* verts, const float* base,
+                                                          int64_t n, float amp, float freq,
+                                                          float phase) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float x = base[3 * i + 0];
+  verts[3 * i + 0] = x;
+  verts[3 * i + 1] = base[3 * i + 1] + amp * sinf(freq * x + phase);
+  verts[3 * i + 2] = base[3 * i + 2];
+}
+
+}  // namespace
+
+int launch_prim_setup(const DevRefit& r, void* stream) {
+  if (r.n_slots <= 0) return 0;
+  const unsigned blocks = (unsigned)((r.n_slots + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_prim_setup, dim3(blocks), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), r);
+  return (int)hipGetLastError();
+}
+
+int launch_refit(const DevRefit& r, void* stream) {
+  if (r.n_leaves <= 0) return 0;
+  const unsigned blocks = (unsigned)((r.n_leaves + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), r);
+  return (int)hipGetLastError();
+}
+
+int launch_displace_sine(float* verts, const float* base, int64_t n_verts, float amp, float freq,
+                         float phase, void* stream) {
+  if (n_verts <= 0) return 0;
+  const unsigned blocks = (unsigned)((n_verts + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_displace_sine, dim3(blocks), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), verts, base, n_verts, amp, freq,
+                     phase);
+  return (int)hipGetLastError();
+}
+
+}  // namespace mh

@@ -1,0 +1,521 @@
+// Traversal kernels for gfx950 (wave64).
+//
+// k_trace_ray: one ray per lane, DynBVH::intersect on a one-ray packet
+//   (Model/Groups/DynBVH.cc:101-404). Inner nodes are 64-byte PairNodes that hold both
+//   child boxes, so one fetch tests both children; the child the reference visits first
+//   (direction sign on the split axis for closest-hit, isLeftCheaper for any-hit,
+//   DynBVH.cc:386-398) is entered and the other is pushed with its entry distance. A
+//   popped entry is entered only if that distance is still <= the ray's minT, which is
+//   exactly the reference's later slab test of that child with the updated minT. The
+//   traversal stack is a per-lane ring of `S` entries in LDS ([slot][lane], conflict free)
+//   whose oldest entries spill to a per-thread global area when the ring is full.
+//
+// k_trace_packet: one RayPacket of up to 64 consecutive rays per wave, the reference's
+//   packet recursion (intersectNode/firstIntersects/lastIntersects, DynBVH.cc:274-905,
+//   scalar branch). Node records are read with uniform (scalar) loads, each lane tests
+//   its own ray, and ballots give firstActive/lastActive. The interval-arithmetic
+//   frustum test (DynBVH.cc:682-721) runs on wave-reduced ray bounds. The wave-uniform
+//   stack of (node, firstActive) lives in LDS.
+//
+// Arithmetic follows the reference's scalar build operation for operation: no
+// contraction (the library is compiled with -ffp-contract=off), IEEE division for
+// 1/direction and the triangle reciprocal, and the reference's select-based min/max.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "mh_device.h"
+
+namespace mh {
+namespace {
+
+constexpr float kEps = 1.e-4f;   // T_EPSILON
+constexpr float kMaxT = 1.e19f;  // MAXT
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kPacketStack = 128;
+
+struct Ray {
+  float o[3], d[3], inv[3];
+};
+
+// Slab test of DynBVH::firstIntersects (scalar loop, DynBVH.cc:666-680).
+__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz,
+                                     const Ray& r, float minT, float& tnear) {
+  float tmin = kEps, tmax = minT;
+  float t0 = (lx - r.o[0]) * r.inv[0];
+  float t1 = (hx - r.o[0]) * r.inv[0];
+  float nr = (t0 < t1) ? t0 : t1;
+  float fr = (t0 < t1) ? t1 : t0;
+  tmin = (tmin < nr) ? nr : tmin;
+  tmax = (fr < tmax) ? fr : tmax;
+  t0 = (ly - r.o[1]) * r.inv[1];
+  t1 = (hy - r.o[1]) * r.inv[1];
+  nr = (t0 < t1) ? t0 : t1;
+  fr = (t0 < t1) ? t1 : t0;
+  tmin = (tmin < nr) ? nr : tmin;
+  tmax = (fr < tmax) ? fr : tmax;
+  t0 = (lz - r.o[2]) * r.inv[2];
+  t1 = (hz - r.o[2]) * r.inv[2];
+  nr = (t0 < t1) ? t0 : t1;
+  fr = (t0 < t1) ? t1 : t0;
+  tmin = (tmin < nr) ? nr : tmin;
+  tmax = (fr < tmax) ? fr : tmax;
+  tnear = tmin;
+  return tmin <= tmax;
+}
+
+// KenslerShirleyTriangle::intersect, scalar branch (KenslerShirleyTriangle.cc:336-377),
+// then RayPacket::hit (Interface/RayPacket.h:531-542). Slot data: p0, e0, e1, n.
+__device__ __forceinline__ bool ks_test(const float4 A, const float4 B, const float4 C,
+                                        const Ray& r, float minT, float& t, float& u, float& v) {
+  const float n0 = C.y, n1 = C.z, n2 = C.w;
+  const float det = n0 * r.d[0] + n1 * r.d[1] + n2 * r.d[2];
+  const float rcp = 1.0f / det;
+  const float g0 = A.x - r.o[0], g1 = A.y - r.o[1], g2 = A.z - r.o[2];
+  const float tt = (n0 * g0 + n1 * g1 + n2 * g2) * rcp;
+  if (tt > minT - kEps || tt < kEps) return false;
+  const float i0 = g1 * r.d[2] - g2 * r.d[1];
+  const float i1 = g2 * r.d[0] - g0 * r.d[2];
+  const float i2 = g0 * r.d[1] - g1 * r.d[0];
+  const float uu = (i0 * B.z + i1 * B.w + i2 * C.x) * rcp;
+  if (uu < 0.0f) return false;
+  const float vv = (i0 * A.w + i1 * B.x + i2 * B.y) * rcp;
+  if (uu + vv > 1.0f || vv < 0.0f) return false;
+  if (!(tt > kEps && tt < minT)) return false;
+  t = tt;
+  u = uu;
+  v = vv;
+  return true;
+}
+
+// WaldTriangle::intersect, scalar branch (WaldTriangle.cc:489-555), then RayPacket::hit.
+__device__ __forceinline__ bool wald_test(const float4 A, const float4 B, const float4 C,
+                                          const Ray& r, float minT, float& t, float& u,
+                                          float& v) {
+  // Projection axis k and its successors ku = (k+1)%3, kv = (k+2)%3, chosen with
+  // bit masks so that no per-lane indexed (scratch) access is generated.
+  const int k = __float_as_int(A.w);
+  const unsigned m0 = 0u - (unsigned)(k == 0), m1 = 0u - (unsigned)(k == 1),
+                 m2 = 0u - (unsigned)(k == 2);
+  auto pick = [&](float a, float b, float c) {
+    return __uint_as_float((__float_as_uint(a) & m0) | (__float_as_uint(b) & m1) |
+                           (__float_as_uint(c) & m2));
+  };
+  const float dk = pick(r.d[0], r.d[1], r.d[2]);
+  const float dku = pick(r.d[1], r.d[2], r.d[0]);
+  const float dkv = pick(r.d[2], r.d[0], r.d[1]);
+  const float ok = pick(r.o[0], r.o[1], r.o[2]);
+  const float oku = pick(r.o[1], r.o[2], r.o[0]);
+  const float okv = pick(r.o[2], r.o[0], r.o[1]);
+  const float nd0 = A.x * dku + A.y * dkv + dk;
+  const float nd = 1.f / nd0;
+  const float f0 = A.z - (ok + A.x * oku + A.y * okv);
+  const float f = f0 * nd;
+  if (f < kEps || f > minT) return false;
+  const float hu = oku + f * dku;
+  const float hv = okv + f * dkv;
+  const float lambda = B.z + hu * B.x + hv * B.y;
+  if (lambda < 0.f) return false;
+  const float mue = C.z + hu * C.x + hv * C.y;
+  if (mue < 0.f || mue + lambda > 1.f) return false;
+  if (!(f > kEps && f < minT)) return false;
+  t = f;
+  u = lambda;
+  v = mue;
+  return true;
+}
+
+template <int PRIM>
+__device__ __forceinline__ bool prim_test(const float* tris, int slot, const Ray& r, float minT,
+                                          float& t, float& u, float& v) {
+  const float4* p = reinterpret_cast<const float4*>(tris + 12 * (size_t)slot);
+  const float4 A = p[0], B = p[1], C = p[2];
+  if (PRIM == 0) return ks_test(A, B, C, r, minT, t, u, v);
+  return wald_test(A, B, C, r, minT, t, u, v);
+}
+
+__device__ __forceinline__ void load_ray(const DevRays& rays, int64_t i, Ray& r) {
+  r.o[0] = rays.ox[i];
+  r.o[1] = rays.oy[i];
+  r.o[2] = rays.oz[i];
+  r.d[0] = rays.dx[i];
+  r.d[1] = rays.dy[i];
+  r.d[2] = rays.dz[i];
+  // RayPacket::actualComputeInverseDirections, scalar branch (RayPacket.cc:105-109)
+  r.inv[0] = 1.0f / r.d[0];
+  r.inv[1] = 1.0f / r.d[1];
+  r.inv[2] = 1.0f / r.d[2];
+}
+
+__device__ __forceinline__ void store_hit(const DevHits& h, int64_t i, int32_t prim, float t,
+                                          float u, float v, bool occ) {
+  if (h.prim) h.prim[i] = prim;
+  if (h.t) h.t[i] = t;
+  if (h.u) h.u[i] = u;
+  if (h.v) h.v[i] = v;
+  if (h.occluded) h.occluded[i] = occ ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Single-ray traversal
+// ---------------------------------------------------------------------------
+template <bool ANY, int PRIM, int S, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_trace_ray(DevScene sc, DevRays rays, DevHits hits,
+                                                      int64_t n, int2* spill,
+                                                      int64_t spill_stride,
+                                                      unsigned long long* counters) {
+  static_assert((S & (S - 1)) == 0, "ring size is a power of two");
+  __shared__ int s_ref[S * kBlock];
+  __shared__ float s_tn[S * kBlock];
+  const int tid = threadIdx.x;
+  const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  unsigned long long nbox = 0, ntri = 0;
+  const RefNode root = sc.refs[0];
+
+  for (int64_t ri = gtid; ri < n; ri += stride) {
+    if (rays.masked && rays.masked[ri]) {
+      store_hit(hits, ri, -2, -kMaxT, 0.f, 0.f, true);
+      continue;
+    }
+    Ray r;
+    load_ray(rays, ri, r);
+    float minT = rays.tmax[ri];
+    int best = -1;
+    float bu = 0.f, bv = 0.f;
+    float tn;
+    bool go = slab(root.box.lo[0], root.box.hi[0], root.box.lo[1], root.box.hi[1],
+                   root.box.lo[2], root.box.hi[2], r, minT, tn);
+    if (STATS) ++nbox;
+    int cur = sc.root_ref;
+    int sp = 0, base = 0;
+    while (go) {
+      if (cur >= 0) {
+        const float4* np = reinterpret_cast<const float4*>(sc.pairs + cur);
+        const float4 A = np[0], B = np[1], C = np[2];
+        const int4 M = reinterpret_cast<const int4*>(np)[3];
+        float tn0, tn1;
+        const bool h0 = slab(A.x, A.y, A.z, A.w, B.x, B.y, r, minT, tn0);
+        const bool h1 = slab(B.z, B.w, C.x, C.y, C.z, C.w, r, minT, tn1);
+        if (STATS) nbox += 2;
+        bool one_first;
+        if (ANY) {
+          one_first = ((M.z >> 2) & 1) == 0;
+        } else {
+          const int ax = M.z & 3;
+          const float da = ax == 0 ? r.d[0] : (ax == 1 ? r.d[1] : r.d[2]);
+          one_first = !(da > 0.f);
+        }
+        const int fref = one_first ? M.y : M.x;
+        const int bref = one_first ? M.x : M.y;
+        const bool fh = one_first ? h1 : h0;
+        const bool bh = one_first ? h0 : h1;
+        const float btn = one_first ? tn0 : tn1;
+        if (fh) {
+          if (bh) {
+            if (sp - base == S) {
+              const int sl = base & (S - 1);
+              spill[(int64_t)base * spill_stride + gtid] =
+                  make_int2(s_ref[sl * kBlock + tid], __float_as_int(s_tn[sl * kBlock + tid]));
+              ++base;
+            }
+            const int sl = sp & (S - 1);
+            s_ref[sl * kBlock + tid] = bref;
+            s_tn[sl * kBlock + tid] = btn;
+            ++sp;
+          }
+          cur = fref;
+          continue;
+        }
+        if (bh) {
+          cur = bref;
+          continue;
+        }
+      } else {
+        const int code = ~cur;
+        const int first = code >> kLeafCountBits;
+        int cnt = code & kLeafCountInline;
+        if (cnt == kLeafCountInline) cnt = sc.big_count[first];
+        bool done = false;
+        for (int k = 0; k < cnt; ++k) {
+          if (STATS) ++ntri;
+          float t, u, v;
+          if (prim_test<PRIM>(sc.tris, first + k, r, minT, t, u, v)) {
+            minT = t;
+            best = first + k;
+            bu = u;
+            bv = v;
+            if (ANY) {
+              done = true;
+              break;
+            }
+          }
+        }
+        if (done) break;
+      }
+      go = false;
+      while (sp > 0) {
+        int ref;
+        float t;
+        if (sp == base) {
+          --sp;
+          --base;
+          const int2 e = spill[(int64_t)sp * spill_stride + gtid];
+          ref = e.x;
+          t = __int_as_float(e.y);
+        } else {
+          --sp;
+          const int sl = sp & (S - 1);
+          ref = s_ref[sl * kBlock + tid];
+          t = s_tn[sl * kBlock + tid];
+        }
+        if (t <= minT) {
+          cur = ref;
+          go = true;
+          break;
+        }
+      }
+    }
+    const bool hit = best >= 0;
+    store_hit(hits, ri, hit ? sc.obj_ids[best] : -1, (ANY && hit) ? kEps : minT, bu, bv, hit);
+  }
+  if (STATS && counters) {
+    atomicAdd(counters + 0, nbox);
+    atomicAdd(counters + 1, ntri);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Packet traversal (one RayPacket per wave)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_min(float x) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float y = __shfl_xor(x, m, 64);
+    x = (x < y) ? x : y;
+  }
+  return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float y = __shfl_xor(x, m, 64);
+    x = (x < y) ? y : x;
+  }
+  return x;
+}
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int lowest(unsigned long long m) { return __builtin_ctzll(m); }
+__device__ __forceinline__ int highest(unsigned long long m) { return 63 - __builtin_clzll(m); }
+__device__ __forceinline__ unsigned long long above(int b) {
+  return b >= 63 ? 0ull : (~0ull << (b + 1));
+}
+
+struct IA {
+  float min_rcp[3], max_rcp[3], min_org[3], max_org[3];
+};
+
+// Interval-arithmetic frustum exit of the scalar firstIntersects (DynBVH.cc:682-721).
+__device__ __forceinline__ bool frustum_miss(const RefNode& nd, const IA& ia) {
+  float tmin_f = 1e-5f, tmax_f = FLT_MAX;
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    float a = nd.box.lo[axis] - ia.max_org[axis];
+    float b = nd.box.hi[axis] - ia.min_org[axis];
+    const float ar0 = a * ia.min_rcp[axis];
+    const float ar1 = a * ia.max_rcp[axis];
+    const float br0 = b * ia.min_rcp[axis];
+    const float br1 = b * ia.max_rcp[axis];
+    a = (br0 < br1) ? br0 : br1;
+    a = (a < ar1) ? a : ar1;
+    a = (a < ar0) ? a : ar0;
+    b = (br0 < br1) ? br1 : br0;
+    b = (b < ar1) ? ar1 : b;
+    b = (b < ar0) ? ar0 : b;
+    tmin_f = (tmin_f < a) ? a : tmin_f;
+    tmax_f = (tmax_f > b) ? b : tmax_f;
+  }
+  return tmin_f > tmax_f;
+}
+
+template <bool ANY, int PRIM, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_trace_packet(DevScene sc, DevRays rays,
+                                                         DevHits hits, int64_t n, int psize,
+                                                         unsigned long long* counters) {
+  __shared__ int2 stk[kWaves][kPacketStack];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t pk = (int64_t)blockIdx.x * kWaves + wv;
+  const int64_t first = pk * psize;
+  if (first >= n) return;
+  const int cnt = (int)((n - first) < psize ? (n - first) : psize);
+  const bool in = lane < cnt;
+  const int64_t ri = first + lane;
+
+  Ray r;
+  float minT;
+  int st;  // 0 not hit, 1 hit, -1 masked, -3 outside the packet
+  if (in) {
+    load_ray(rays, ri, r);
+    const bool m = rays.masked && rays.masked[ri];
+    st = m ? -1 : 0;
+    minT = m ? -kMaxT : rays.tmax[ri];
+  } else {
+    r.o[0] = r.o[1] = r.o[2] = 0.f;
+    r.d[0] = r.d[1] = r.d[2] = 1.f;
+    r.inv[0] = r.inv[1] = r.inv[2] = 1.f;
+    st = -3;
+    minT = -kMaxT;
+  }
+  // DynBVH::intersect IAData over the packet's unmasked rays (DynBVH.cc:246-266)
+  IA ia;
+  const bool contrib = st == 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ia.min_rcp[a] = wave_min(contrib ? r.inv[a] : FLT_MAX);
+    ia.max_rcp[a] = wave_max(contrib ? r.inv[a] : -FLT_MAX);
+    ia.min_org[a] = wave_min(contrib ? r.o[a] : FLT_MAX);
+    ia.max_org[a] = wave_max(contrib ? r.o[a] : -FLT_MAX);
+  }
+
+  int pos = -1;
+  float hu = 0.f, hv = 0.f;
+  unsigned long long nbox = 0, ntri = 0;
+  int sp = 0;
+  stk[wv][sp++] = make_int2(0, 0);
+  while (sp > 0) {
+    --sp;
+    const int2 e = stk[wv][sp];
+    const int nid = uni(e.x);
+    const int b = uni(e.y);
+    const RefNode nd = sc.refs[nid];
+    const bool elig = in && lane >= b && st != -1 && !(ANY && st == 1);
+    const unsigned long long E = __ballot(elig);
+    if (E == 0) continue;
+    const int rb = lowest(E);
+    float tn;
+    const bool hb = elig && slab(nd.box.lo[0], nd.box.hi[0], nd.box.lo[1], nd.box.hi[1],
+                                 nd.box.lo[2], nd.box.hi[2], r, minT, tn);
+    const unsigned long long H = __ballot(hb);
+    if (!((H >> rb) & 1ull)) {
+      if (frustum_miss(nd, ia)) {
+        if (STATS) nbox += 1;
+        continue;
+      }
+    }
+    if (H == 0) {
+      if (STATS) nbox += __popcll(E);
+      continue;
+    }
+    const int fa = lowest(H);
+    if (STATS) nbox += __popcll(E & (fa >= 63 ? ~0ull : ((2ull << fa) - 1)));
+    if (nd.children != 0) {
+      const unsigned long long Lm = H & above(b);
+      const int la = Lm ? highest(Lm) : b;
+      int sb = fa, se = la + 1;
+      for (int k = 0; k < nd.children; ++k) {
+        const int slot = nd.child + k;
+        if (STATS) ntri += (unsigned long long)(se - sb);
+        const bool act = lane >= sb && lane < se;
+        if (act) {
+          float t, u, v;
+          if (prim_test<PRIM>(sc.tris, slot, r, minT, t, u, v)) {
+            minT = t;
+            st = 1;
+            pos = slot;
+            hu = u;
+            hv = v;
+          }
+        }
+        if (ANY) {
+          // DynBVH.cc:329-374: opaque hits end the ray (minT = T_EPSILON); the
+          // subpacket shrinks to the rays still unoccluded.
+          const bool hit_here = act && st == 1;
+          if (hit_here) minT = kEps;
+          if (__ballot(hit_here)) {
+            const unsigned long long alive = __ballot(act && st == 0);
+            if (!alive) break;
+            sb = lowest(alive);
+            se = highest(alive) + 1;
+          }
+        }
+      }
+    } else {
+      int front;
+      if (ANY) {
+        front = (nd.bits & 1) ? 0 : 1;
+      } else {
+        const float da = nd.axis == 0 ? r.d[0] : (nd.axis == 1 ? r.d[1] : r.d[2]);
+        front = (__shfl(da, fa, 64) > 0.f) ? 0 : 1;
+      }
+      stk[wv][sp++] = make_int2(nd.child + 1 - front, fa);
+      stk[wv][sp++] = make_int2(nd.child + front, fa);
+    }
+  }
+  if (in) {
+    const int32_t prim = st == 1 ? sc.obj_ids[pos] : (st == -1 ? -2 : -1);
+    store_hit(hits, ri, prim, minT, hu, hv, st != 0);
+  }
+  if (STATS && counters && lane == 0) {
+    atomicAdd(counters + 0, nbox);
+    atomicAdd(counters + 1, ntri);
+  }
+}
+
+template <bool ANY, int PRIM, bool STATS>
+int launch_ray_s(const DevScene& sc, const DevRays& rays, const DevHits& hits, int64_t n,
+                 const LaunchCfg& cfg, hipStream_t s) {
+  unsigned long long* c = reinterpret_cast<unsigned long long*>(cfg.counters);
+  int2* sp = reinterpret_cast<int2*>(cfg.spill);
+  dim3 g(cfg.blocks), b(kBlock);
+  switch (cfg.stack_lds) {
+    case 8:
+      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, 8, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
+                         cfg.spill_stride, c);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, 32, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
+                         cfg.spill_stride, c);
+      break;
+    default:
+      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, 16, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
+                         cfg.spill_stride, c);
+      break;
+  }
+  return (int)hipGetLastError();
+}
+
+template <bool ANY, int PRIM>
+int launch_any_prim(const DevScene& sc, const DevRays& rays, const DevHits& hits, int64_t n,
+                    const LaunchCfg& cfg, hipStream_t s) {
+  const bool stats = cfg.counters != nullptr;
+  if (cfg.packet_size > 1) {
+    const int64_t packets = (n + cfg.packet_size - 1) / cfg.packet_size;
+    const int64_t blocks = (packets + kWaves - 1) / kWaves;
+    unsigned long long* c = reinterpret_cast<unsigned long long*>(cfg.counters);
+    if (stats)
+      hipLaunchKernelGGL((k_trace_packet<ANY, PRIM, true>), dim3((unsigned)blocks), dim3(kBlock),
+                         0, s, sc, rays, hits, n, cfg.packet_size, c);
+    else
+      hipLaunchKernelGGL((k_trace_packet<ANY, PRIM, false>), dim3((unsigned)blocks),
+                         dim3(kBlock), 0, s, sc, rays, hits, n, cfg.packet_size, c);
+    return (int)hipGetLastError();
+  }
+  return stats ? launch_ray_s<ANY, PRIM, true>(sc, rays, hits, n, cfg, s)
+               : launch_ray_s<ANY, PRIM, false>(sc, rays, hits, n, cfg, s);
+}
+
+}  // namespace
+
+int launch_trace(const DevScene& sc, const DevRays& rays, const DevHits& hits, int64_t n,
+                 const LaunchCfg& cfg, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (cfg.any_hit) {
+    return sc.prim_kind == 1 ? launch_any_prim<true, 1>(sc, rays, hits, n, cfg, s)
+                             : launch_any_prim<true, 0>(sc, rays, hits, n, cfg, s);
+  }
+  return sc.prim_kind == 1 ? launch_any_prim<false, 1>(sc, rays, hits, n, cfg, s)
+                           : launch_any_prim<false, 0>(sc, rays, hits, n, cfg, s);
+}
+
+}  // namespace mh
