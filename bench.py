@@ -54,6 +54,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--bpc", type=int, default=0, help="blocks per CU for single-ray kernels")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -158,6 +160,22 @@ def main() -> int:
                 "bytes_per_ray": round(bytes_ray_b, 1), "rays_per_launch": n_b,
                 "launch_ms": round(k_b, 4)}
 
+    if args.copy_probe_mb > 0:
+        # A copy kernel of known size in the same process: the reference for scaling
+        # FETCH_SIZE / WRITE_SIZE counter readings to bytes.
+        nbytes = args.copy_probe_mb << 20
+        a, b = api.DeviceArray(nbytes), api.DeviceArray(nbytes)
+        for _ in range(3):
+            L.check(L.load().mh_device_copy(b.ptr, a.ptr, nbytes, stream))
+        api.sync(stream)
+        log(f"copy probe: 3 x {nbytes} bytes")
+        a.free()
+        b.free()
+    traffic = load_traffic()
+    if traffic is not None:
+        roofline["traffic"] = traffic["bytes_per_launch"]
+        roofline["traffic_source"] = traffic["source"]
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(verts, idx, scene, prim_rays, bounce)
@@ -183,6 +201,15 @@ def main() -> int:
     return 0
 
 
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed counter summary."""
+    p = REPO / "profiles" / "traffic_trace_ray_bounce_heightfield1M.json"
+    if not p.exists():
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
 def cpu_baseline(verts, idx, scene, prim_rays, bounce) -> dict:
     """The oracle's packet traversal (8x8 packets, one thread per core) on a sample."""
     sys.path.insert(0, str(REPO / "oracle"))
@@ -192,25 +219,25 @@ def cpu_baseline(verts, idx, scene, prim_rays, bounce) -> dict:
     cores = host_cores()
     total_rays = 0
     total_s = 0.0
-    sample = {}
-    for name, rays, budget in (("primary", prim_rays, 6.0), ("bounce", bounce, 6.0)):
-        n = 64 * 1024
-        while True:
-            sub = {k: rays[k][:n] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
+    reps = {}
+    for name, rays, budget in (("primary", prim_rays, 5.0), ("bounce", bounce, 5.0)):
+        sub = {k: rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
+        n = len(sub["ox"])
+        done, spent = 0, 0.0
+        while spent < budget or done < 2:
             t0 = time.perf_counter()
             pyoracle.trace(verts, idx, nodes, ids, sub, packet_size=64, threads=cores)
-            dt = time.perf_counter() - t0
-            if dt >= budget or n >= len(rays["ox"]):
-                break
-            n = min(len(rays["ox"]), int(n * max(2.0, 1.2 * budget / max(dt, 1e-3))))
-        sample[name] = n
-        total_rays += n
-        total_s += dt
-        log(f"cpu baseline {name}: {n} rays in {dt:.2f}s on {cores} threads")
+            spent += time.perf_counter() - t0
+            done += 1
+        reps[name] = done
+        total_rays += n * done
+        total_s += spent
+        log(f"cpu baseline {name}: {done} x {n} rays in {spent:.2f}s on {cores} threads")
     return {"value": round(total_rays / total_s / 1e6, 3), "unit": "Mrays/s", "cores": cores,
             "kind": "port",
-            "sample": f"first {sample['primary']} primary (8x8 tile order) and first "
-                      f"{sample['bounce']} bounce rays, 64-ray packets, closest-hit"}
+            "sample": f"the full frame's {len(prim_rays['ox'])} primary rays x{reps['primary']} "
+                      f"and {len(bounce['ox'])} bounce rays x{reps['bounce']}, 64-ray packets "
+                      f"in 8x8-tile order, closest-hit, ~10 s of CPU time"}
 
 
 if __name__ == "__main__":

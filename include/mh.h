@@ -174,6 +174,8 @@ int mh_malloc(void** ptr, size_t bytes);
 int mh_free(void* ptr);
 int mh_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int mh_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a streaming kernel (16-byte vector loads/stores) on `stream`. */
+int mh_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 int mh_stream_create(void** stream);
 int mh_stream_destroy(void* stream);
 int mh_stream_sync(void* stream);

@@ -972,6 +972,22 @@ int oracle_build(const float* verts, int nverts, const uint32_t* idx, int ntris,
   return b.num_nodes;
 }
 
+// TreeTraversalProb::computeProbs for one parent/left/right triple; boxes are
+// lo.xyz, hi.xyz. out3 = {lr, jl, jr}.
+void oracle_ttp_probs(const float* parent6, const float* left6, const float* right6,
+                      float* out3) {
+  auto mk = [](const float* b) {
+    Box x;
+    for (int i = 0; i < 3; ++i) {
+      x.lo[i] = b[i];
+      x.hi[i] = b[3 + i];
+    }
+    return x;
+  };
+  TTP ttp;
+  ttp.probs(mk(parent6), mk(left6), mk(right6), out3[0], out3[1], out3[2]);
+}
+
 // Recomputes isLeftCheaper on an existing tree (DynBVH::computeTraversalCost).
 void oracle_rtsah(void* nodes, int n_nodes) {
   std::vector<Node32> nv((Node32*)nodes, (Node32*)nodes + n_nodes);

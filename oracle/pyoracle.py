@@ -49,6 +49,8 @@ def load() -> C.CDLL:
         lib.oracle_wald_setup.restype = None
         lib.oracle_wald_setup.argtypes = [P, P, C.c_int, P]
         lib.oracle_node_size.restype = C.c_int
+        lib.oracle_ttp_probs.restype = None
+        lib.oracle_ttp_probs.argtypes = [P, P, P, P]
         _lib = lib
     return _lib
 
@@ -133,6 +135,14 @@ def refit(verts, idx, nodes, ids):
     assert nodes.flags["C_CONTIGUOUS"]
     load().oracle_refit(_p(v), len(v), _p(i), len(i), _p(nodes),
                         _p(np.ascontiguousarray(ids, np.int32)))
+
+
+def ttp_probs(parent, left, right):
+    """(lr, jl, jr) of TreeTraversalProb::computeProbs; boxes as (lo.xyz, hi.xyz)."""
+    a = [np.ascontiguousarray(x, np.float32) for x in (parent, left, right)]
+    out = np.empty(3, np.float32)
+    load().oracle_ttp_probs(*[_p(x) for x in a], _p(out))
+    return out
 
 
 def wald_setup(verts, idx):

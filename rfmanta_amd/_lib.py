@@ -85,6 +85,7 @@ _SIGS = {
     "mh_free": (C.c_int, [_P]),
     "mh_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
     "mh_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+    "mh_device_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "mh_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "mh_stream_destroy": (C.c_int, [_P]),
     "mh_stream_sync": (C.c_int, [_P]),

@@ -668,6 +668,14 @@ int mh_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return MH_OK;
 }
 
+int mh_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if ((!dst || !src) && bytes) return fail(MH_E_INVALID, "null argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIP_OK((hipError_t)mh::launch_copy(dst, src, bytes, g_lib.cu_count, stream));
+  return MH_OK;
+}
+
 int mh_stream_create(void** stream) {
   if (!stream) return fail(MH_E_INVALID, "null argument");
   int rc = ensure_init();

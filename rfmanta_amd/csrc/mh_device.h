@@ -91,4 +91,7 @@ int launch_refit(const DevRefit& r, void* stream);       // tris + bounds, botto
 int launch_displace_sine(float* verts, const float* base, int64_t n_verts, float amp, float freq,
                          float phase, void* stream);
 
+// util.hip: streaming copy with 16-byte vector accesses (plus a byte tail).
+int launch_copy(void* dst, const void* src, size_t bytes, int cu_count, void* stream);
+
 }  // namespace mh
