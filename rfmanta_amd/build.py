@@ -64,7 +64,23 @@ def build(verbose: bool = False, save_temps: bool = False) -> Path:
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs,
           "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    build_manta_harness(verbose)
     return LIB
+
+
+MANTA = PKG / "manta"
+HARNESS = MANTA / "libmanta_harness.so"
+
+
+def build_manta_harness(verbose: bool = False) -> Path:
+    """HipDynBVH (the Manta AccelerationStructure over the C ABI) and its driver."""
+    inc = [f"-I{MANTA / 'compat'}", f"-I{MANTA}", f"-I{REPO / 'include'}"]
+    tmp = HARNESS.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc,
+          str(MANTA / "HipDynBVH.cc"), str(MANTA / "manta_harness.cc"), "-o", str(tmp),
+          f"-L{PKG}", "-l:libmh.so", "-Wl,-rpath,$ORIGIN/..", "-pthread"], verbose)
+    os.replace(tmp, HARNESS)
+    return HARNESS
 
 
 if __name__ == "__main__":

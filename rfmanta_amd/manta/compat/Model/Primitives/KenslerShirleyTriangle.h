@@ -1,0 +1,2 @@
+// Forwarding header: KenslerShirleyTriangle is declared with MeshTriangle in this subset.
+#include <Model/Primitives/MeshTriangle.h>
