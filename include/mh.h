@@ -169,6 +169,53 @@ int mh_trace(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, in
 int mh_trace_host(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
                   uint32_t flags, const mh_trace_opts* opts);
 
+/* ---- Persistent multi-batch frames ------------------------------------------------
+ * No reference counterpart: Manta traces one 64-ray packet per call. A frame is a fixed
+ * table of ray batches (uneven sizes, each closest-hit or any-hit, device-resident SoA
+ * arrays) traced by one persistent launch in which every wave refills finished lanes
+ * from a shared ray counter. Per-ray results equal mh_trace with packet_size 1
+ * (DynBVH::intersect on one-ray packets). A frame can be captured once into a HIP graph
+ * and replayed; the batch table may be rewritten between replays (same batch count). */
+#define MH_LAYOUT_PAIR 0 /* 64-byte nodes holding both child boxes */
+#define MH_LAYOUT_REF 1  /* the reference's 32-byte BVHNode records */
+#define MH_MAX_BATCHES 256
+
+typedef struct mh_batch {
+  mh_rays rays;
+  mh_hits hits;
+  int64_t count;
+  uint32_t flags; /* MH_CLOSEST_HIT or MH_ANY_HIT */
+} mh_batch;
+
+typedef struct mh_launch_config {
+  int layout;        /* MH_LAYOUT_* */
+  int stack_lds;     /* per-lane LDS stack ring: 8, 16 or 32 entries */
+  int block;         /* lanes per workgroup: 64, 128 or 256 */
+  int blocks_per_cu; /* persistent workgroups per compute unit */
+  int refill;        /* idle lanes of a wave that trigger a refill, 1..64 */
+} mh_launch_config;
+
+typedef struct mh_frame mh_frame;
+
+int mh_frame_create(const mh_scene* scene, const mh_batch* batches, int n_batches,
+                    mh_frame** out);
+/* Rewrites the batch table (n_batches must equal the frame's), on `stream`. */
+int mh_frame_set_batches(mh_frame* frame, const mh_batch* batches, int n_batches, void* stream);
+int mh_frame_set_config(mh_frame* frame, const mh_launch_config* cfg); /* NULL = default */
+int mh_frame_get_config(const mh_frame* frame, mh_launch_config* cfg);
+int mh_frame_launch(mh_frame* frame, void* stream);
+/* Captures one launch of the current configuration into a HIP graph (stream capture). */
+int mh_frame_capture(mh_frame* frame, void* stream);
+int mh_frame_replay(mh_frame* frame, void* stream);
+/* Average milliseconds per launch over `iters` launches (graph replays when use_graph). */
+int mh_frame_time(mh_frame* frame, int iters, int use_graph, void* stream, float* ms);
+/* Sweeps layout x stack x block x blocks_per_cu x refill, keeps the fastest and stores it
+ * in `cache_path` (JSON, may be NULL) keyed by the scene and batch mix; a cached entry
+ * is reused without sweeping unless `force`. */
+int mh_frame_autotune(mh_frame* frame, int iters, const char* cache_path, int force,
+                      mh_launch_config* best, float* best_ms);
+int mh_frame_destroy(mh_frame* frame);
+
 /* Device memory helpers for callers without their own allocator. */
 int mh_malloc(void** ptr, size_t bytes);
 int mh_free(void* ptr);

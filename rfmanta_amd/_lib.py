@@ -50,6 +50,22 @@ class TraceOpts(C.Structure):
                 ("counters", C.c_void_p)]
 
 
+class Batch(C.Structure):
+    _fields_ = [("rays", Rays), ("hits", Hits), ("count", C.c_int64), ("flags", C.c_uint32)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("layout", C.c_int), ("stack_lds", C.c_int), ("block", C.c_int),
+                ("blocks_per_cu", C.c_int), ("refill", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MH_LAYOUT_PAIR = 0
+MH_LAYOUT_REF = 1
+
+
 class BuildOpts(C.Structure):
     _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int)]
 
@@ -81,6 +97,17 @@ _SIGS = {
                            C.POINTER(TraceOpts), _P]),
     "mh_trace_host": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,
                                 C.POINTER(TraceOpts)]),
+    "mh_frame_create": (C.c_int, [_P, C.POINTER(Batch), C.c_int, C.POINTER(C.c_void_p)]),
+    "mh_frame_set_batches": (C.c_int, [_P, C.POINTER(Batch), C.c_int, _P]),
+    "mh_frame_set_config": (C.c_int, [_P, C.POINTER(LaunchConfig)]),
+    "mh_frame_get_config": (C.c_int, [_P, C.POINTER(LaunchConfig)]),
+    "mh_frame_launch": (C.c_int, [_P, _P]),
+    "mh_frame_capture": (C.c_int, [_P, _P]),
+    "mh_frame_replay": (C.c_int, [_P, _P]),
+    "mh_frame_time": (C.c_int, [_P, C.c_int, C.c_int, _P, C.POINTER(C.c_float)]),
+    "mh_frame_autotune": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(LaunchConfig),
+                                    C.POINTER(C.c_float)]),
+    "mh_frame_destroy": (C.c_int, [_P]),
     "mh_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "mh_free": (C.c_int, [_P]),
     "mh_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),

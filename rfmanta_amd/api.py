@@ -241,6 +241,87 @@ class Scene:
         return out
 
 
+class Frame:
+    """A fixed table of ray batches traced by one persistent launch (mh_frame_*)."""
+
+    def __init__(self, scene: Scene, batches: list):
+        """batches: list of (DeviceRays, DeviceHits, any_hit)."""
+        self.scene = scene
+        self._keep = list(batches)
+        arr = self._table(batches)
+        h = C.c_void_p()
+        L.check(L.load().mh_frame_create(scene.h, arr, len(batches), C.byref(h)))
+        self.h = h.value
+
+    @staticmethod
+    def _table(batches):
+        arr = (L.Batch * len(batches))()
+        for i, (dr, dh, any_hit) in enumerate(batches):
+            arr[i].rays = dr.struct()
+            arr[i].hits = dh.struct()
+            arr[i].count = dr.n
+            arr[i].flags = L.MH_ANY_HIT if any_hit else L.MH_CLOSEST_HIT
+        return arr
+
+    def set_batches(self, batches: list, stream=None) -> None:
+        self._keep = list(batches)
+        L.check(L.load().mh_frame_set_batches(self.h, self._table(batches), len(batches), stream))
+
+    def set_config(self, cfg: dict | None) -> None:
+        c = None
+        if cfg is not None:
+            c = L.LaunchConfig(cfg["layout"], cfg["stack_lds"], cfg["block"],
+                               cfg["blocks_per_cu"], cfg["refill"])
+        L.check(L.load().mh_frame_set_config(self.h, C.byref(c) if c is not None else None))
+
+    def config(self) -> dict:
+        c = L.LaunchConfig()
+        L.check(L.load().mh_frame_get_config(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def launch(self, stream=None) -> None:
+        L.check(L.load().mh_frame_launch(self.h, stream))
+
+    def capture(self, stream) -> None:
+        L.check(L.load().mh_frame_capture(self.h, stream))
+
+    def replay(self, stream) -> None:
+        L.check(L.load().mh_frame_replay(self.h, stream))
+
+    def time(self, iters: int, use_graph: bool = False, stream=None) -> float:
+        ms = C.c_float()
+        L.check(L.load().mh_frame_time(self.h, iters, int(use_graph), stream, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, iters: int = 3, cache_path: str | None = None, force: bool = False):
+        c = L.LaunchConfig()
+        ms = C.c_float()
+        L.check(L.load().mh_frame_autotune(self.h, iters, cache_path.encode() if cache_path else None,
+                                           int(force), C.byref(c), C.byref(ms)))
+        return c.as_dict(), ms.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.load().mh_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_create():
+    s = C.c_void_p()
+    L.check(L.load().mh_stream_create(C.byref(s)))
+    return s.value
+
+
+def stream_destroy(s) -> None:
+    L.check(L.load().mh_stream_destroy(s))
+
+
 def build_dynbvh_host(verts: np.ndarray, idx: np.ndarray, rtsah: bool = True,
                       threads: int = 0) -> tuple[np.ndarray, np.ndarray, int]:
     """Host SAH build (no device): (nodes[2T], ids[T], records in use)."""

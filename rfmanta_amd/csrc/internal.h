@@ -1,0 +1,102 @@
+// Library-internal declarations shared by the C-ABI translation units (mh_api.cpp,
+// frame_api.cpp): error recording, device initialisation, device buffers and the scene.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "mh.h"
+#include "mh_device.h"
+#include "sah_builder.h"
+
+namespace mh_internal {
+
+// Records a thread-local message for mh_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...);
+// Selects the library's device (first call queries it); MH_OK or an error code.
+int ensure_init();
+// Compute units and architecture name of the library's device (valid after ensure_init()).
+int cu_count();
+const char* device_name();
+
+#define HIP_OK(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return ::mh_internal::fail(MH_E_HIP, "%s: %s", #call, hipGetErrorString(e_));               \
+  } while (0)
+
+// One device allocation, released with the owner.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int reserve(size_t n) {
+    if (n <= bytes && p) return MH_OK;
+    release();
+    if (n == 0) return MH_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(MH_E_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return MH_OK;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+}  // namespace mh_internal
+
+struct mh_scene {
+  using DevBuf = mh_internal::DevBuf;
+  std::vector<float> verts;
+  std::vector<uint32_t> idx;
+  uint32_t nv = 0, nt = 0;
+  mh::DynBvh bvh;
+  bool built = false;
+  int prim_kind = MH_PRIM_KENSLER_SHIRLEY;
+  int root_ref = 0;
+  uint32_t n_inner = 0, n_leaves = 0, depth = 0;
+
+  DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, parent, slot, leaves, arrive;
+
+  std::mutex spill_mu;  // guards the per-stream spill areas
+  std::mutex stage_mu;  // serialises mh_trace_host calls on the staging buffer
+  std::map<void*, DevBuf*> spill;
+  DevBuf stage;
+  ~mh_scene() {
+    for (auto& kv : spill) delete kv.second;
+  }
+  size_t device_bytes() const {
+    return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
+           d_base.bytes + d_idx.bytes + parent.bytes + slot.bytes + leaves.bytes + arrive.bytes;
+  }
+  mh::DevScene dev() const {
+    mh::DevScene d;
+    d.pairs = pairs.as<const mh::PairNode>();
+    d.refs = refs.as<const mh::RefNode>();
+    d.tris = tris.as<const float>();
+    d.obj_ids = ids.as<const int32_t>();
+    d.big_count = big.as<const int32_t>();
+    d.root_ref = root_ref;
+    d.prim_kind = prim_kind;
+    return d;
+  }
+};

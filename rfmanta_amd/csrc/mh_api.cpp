@@ -18,28 +18,14 @@
 #include <string>
 #include <vector>
 
+#include "internal.h"
 #include "mh_device.h"
 #include "sah_builder.h"
 
+namespace mh_internal {
+
 namespace {
-
 thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-#define HIP_OK(call)                                                                        \
-  do {                                                                                      \
-    hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess) return fail(MH_E_HIP, "%s: %s", #call, hipGetErrorString(e_));    \
-  } while (0)
 
 struct Lib {
   std::mutex mu;
@@ -50,6 +36,17 @@ struct Lib {
   char name[256] = {0};
 };
 Lib g_lib;
+}  // namespace
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
 
 int ensure_init() {
   std::lock_guard<std::mutex> lk(g_lib.mu);
@@ -71,63 +68,16 @@ int ensure_init() {
   return MH_OK;
 }
 
-// One device allocation, released with the owner.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  int reserve(size_t n) {
-    if (n <= bytes && p) return MH_OK;
-    release();
-    if (n == 0) return MH_OK;
-    hipError_t e = hipMalloc(&p, n);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return fail(MH_E_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
-    }
-    bytes = n;
-    return MH_OK;
-  }
-  template <class T>
-  T* as() const {
-    return reinterpret_cast<T*>(p);
-  }
-};
+int cu_count() { return g_lib.cu_count; }
+const char* device_name() { return g_lib.name; }
 
-}  // namespace
+}  // namespace mh_internal
 
-struct mh_scene {
-  std::vector<float> verts;
-  std::vector<uint32_t> idx;
-  uint32_t nv = 0, nt = 0;
-  mh::DynBvh bvh;
-  bool built = false;
-  int prim_kind = MH_PRIM_KENSLER_SHIRLEY;
-  int root_ref = 0;
-  uint32_t n_inner = 0, n_leaves = 0, depth = 0;
-
-  DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, parent, slot, leaves, arrive;
-
-  std::mutex spill_mu;  // guards the per-stream spill areas
-  std::mutex stage_mu;  // serialises mh_trace_host calls on the staging buffer
-  std::map<void*, DevBuf*> spill;
-  DevBuf stage;
-  ~mh_scene() {
-    for (auto& kv : spill) delete kv.second;
-  }
-  size_t device_bytes() const {
-    return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
-           d_base.bytes + d_idx.bytes + parent.bytes + slot.bytes + leaves.bytes + arrive.bytes;
-  }
-};
+using mh_internal::DevBuf;
+using mh_internal::ensure_init;
+using mh_internal::fail;
+using mh_internal::g_err;
+using mh_internal::g_lib;
 
 namespace {
 
@@ -249,21 +199,11 @@ int validate_ids(const std::vector<int32_t>& ids, uint32_t T) {
   return MH_OK;
 }
 
-mh::DevScene dev_scene(const mh_scene* s) {
-  mh::DevScene d;
-  d.pairs = s->pairs.as<const mh::PairNode>();
-  d.refs = s->refs.as<const mh::RefNode>();
-  d.tris = s->tris.as<const float>();
-  d.obj_ids = s->ids.as<const int32_t>();
-  d.big_count = s->big.as<const int32_t>();
-  d.root_ref = s->root_ref;
-  d.prim_kind = s->prim_kind;
-  return d;
-}
+mh::DevScene dev_scene(const mh_scene* s) { return s->dev(); }
 
 int choose_stack(const mh_trace_opts* o) {
   const int v = o ? o->stack_lds : 0;
-  return (v == 8 || v == 16 || v == 32) ? v : 16;
+  return (v == 8 || v == 16 || v == 32) ? v : 8;
 }
 
 int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int64_t n,
@@ -293,7 +233,7 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
       return fail(MH_E_UNSUPPORTED, "tree depth %u exceeds the packet stack (126)", s->depth);
   } else {
     const int S = choose_stack(opts);
-    const int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : 4;
+    const int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : 16;
     const int64_t want = (n + 255) / 256;
     const int64_t cap = (int64_t)g_lib.cu_count * bpc;
     cfg.blocks = (int)std::max<int64_t>(1, std::min(want, cap));

@@ -65,6 +65,45 @@ struct LaunchCfg {
   int64_t spill_stride;
 };
 
+// Node layouts of the persistent batch kernel (batch.hip).
+constexpr int kLayoutPair = 0;  // 64-byte PairNodes, both child boxes per fetch
+constexpr int kLayoutRef = 1;   // 32-byte reference records, own box tested at visit
+
+// One ray batch of a persistent launch: rays [start, start + count) of the launch's
+// global numbering are rays [0, count) of this batch.
+struct BatchDev {
+  DevRays rays;
+  DevHits hits;
+  int64_t count;
+  int64_t start;
+  uint32_t flags;  // bit 0: any-hit
+  uint32_t pad;
+};
+
+// Device-resident batch table: the kernel reads the batch count and total from here, so a
+// captured graph stays valid when the table's contents change.
+struct BatchTable {
+  int64_t total;
+  int32_t nb;
+  int32_t pad;
+  BatchDev batches[1];  // nb entries
+};
+
+struct BatchLaunch {
+  const BatchTable* table;
+  int nb;  // host copy, for argument checks only
+  unsigned long long* counters;  // device u64[2], zero between launches (self-resetting)
+  int32_t* spill;
+  int64_t spill_stride;  // blocks * block
+  int blocks;
+  int block;      // 64, 128 or 256 lanes
+  int stack_lds;  // 8, 16 or 32
+  int layout;     // kLayoutPair or kLayoutRef
+  int refill;     // idle lanes that trigger a refill, 1..64
+};
+
+int launch_trace_batch(const DevScene& sc, const BatchLaunch& bl, void* stream);
+
 // Launchers (trace.hip). Return hipError_t as int.
 int launch_trace(const DevScene& sc, const DevRays& rays, const DevHits& hits, int64_t n,
                  const LaunchCfg& cfg, void* stream);

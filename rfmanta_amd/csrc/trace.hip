@@ -26,137 +26,14 @@
 #include <cfloat>
 
 #include "mh_device.h"
+#include "trace_common.h"
 
 namespace mh {
 namespace {
 
-constexpr float kEps = 1.e-4f;   // T_EPSILON
-constexpr float kMaxT = 1.e19f;  // MAXT
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kPacketStack = 128;
-
-struct Ray {
-  float o[3], d[3], inv[3];
-};
-
-// Slab test of DynBVH::firstIntersects (scalar loop, DynBVH.cc:666-680).
-__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz,
-                                     const Ray& r, float minT, float& tnear) {
-  float tmin = kEps, tmax = minT;
-  float t0 = (lx - r.o[0]) * r.inv[0];
-  float t1 = (hx - r.o[0]) * r.inv[0];
-  float nr = (t0 < t1) ? t0 : t1;
-  float fr = (t0 < t1) ? t1 : t0;
-  tmin = (tmin < nr) ? nr : tmin;
-  tmax = (fr < tmax) ? fr : tmax;
-  t0 = (ly - r.o[1]) * r.inv[1];
-  t1 = (hy - r.o[1]) * r.inv[1];
-  nr = (t0 < t1) ? t0 : t1;
-  fr = (t0 < t1) ? t1 : t0;
-  tmin = (tmin < nr) ? nr : tmin;
-  tmax = (fr < tmax) ? fr : tmax;
-  t0 = (lz - r.o[2]) * r.inv[2];
-  t1 = (hz - r.o[2]) * r.inv[2];
-  nr = (t0 < t1) ? t0 : t1;
-  fr = (t0 < t1) ? t1 : t0;
-  tmin = (tmin < nr) ? nr : tmin;
-  tmax = (fr < tmax) ? fr : tmax;
-  tnear = tmin;
-  return tmin <= tmax;
-}
-
-// KenslerShirleyTriangle::intersect, scalar branch (KenslerShirleyTriangle.cc:336-377),
-// then RayPacket::hit (Interface/RayPacket.h:531-542). Slot data: p0, e0, e1, n.
-__device__ __forceinline__ bool ks_test(const float4 A, const float4 B, const float4 C,
-                                        const Ray& r, float minT, float& t, float& u, float& v) {
-  const float n0 = C.y, n1 = C.z, n2 = C.w;
-  const float det = n0 * r.d[0] + n1 * r.d[1] + n2 * r.d[2];
-  const float rcp = 1.0f / det;
-  const float g0 = A.x - r.o[0], g1 = A.y - r.o[1], g2 = A.z - r.o[2];
-  const float tt = (n0 * g0 + n1 * g1 + n2 * g2) * rcp;
-  if (tt > minT - kEps || tt < kEps) return false;
-  const float i0 = g1 * r.d[2] - g2 * r.d[1];
-  const float i1 = g2 * r.d[0] - g0 * r.d[2];
-  const float i2 = g0 * r.d[1] - g1 * r.d[0];
-  const float uu = (i0 * B.z + i1 * B.w + i2 * C.x) * rcp;
-  if (uu < 0.0f) return false;
-  const float vv = (i0 * A.w + i1 * B.x + i2 * B.y) * rcp;
-  if (uu + vv > 1.0f || vv < 0.0f) return false;
-  if (!(tt > kEps && tt < minT)) return false;
-  t = tt;
-  u = uu;
-  v = vv;
-  return true;
-}
-
-// WaldTriangle::intersect, scalar branch (WaldTriangle.cc:489-555), then RayPacket::hit.
-__device__ __forceinline__ bool wald_test(const float4 A, const float4 B, const float4 C,
-                                          const Ray& r, float minT, float& t, float& u,
-                                          float& v) {
-  // Projection axis k and its successors ku = (k+1)%3, kv = (k+2)%3, chosen with
-  // bit masks so that no per-lane indexed (scratch) access is generated.
-  const int k = __float_as_int(A.w);
-  const unsigned m0 = 0u - (unsigned)(k == 0), m1 = 0u - (unsigned)(k == 1),
-                 m2 = 0u - (unsigned)(k == 2);
-  auto pick = [&](float a, float b, float c) {
-    return __uint_as_float((__float_as_uint(a) & m0) | (__float_as_uint(b) & m1) |
-                           (__float_as_uint(c) & m2));
-  };
-  const float dk = pick(r.d[0], r.d[1], r.d[2]);
-  const float dku = pick(r.d[1], r.d[2], r.d[0]);
-  const float dkv = pick(r.d[2], r.d[0], r.d[1]);
-  const float ok = pick(r.o[0], r.o[1], r.o[2]);
-  const float oku = pick(r.o[1], r.o[2], r.o[0]);
-  const float okv = pick(r.o[2], r.o[0], r.o[1]);
-  const float nd0 = A.x * dku + A.y * dkv + dk;
-  const float nd = 1.f / nd0;
-  const float f0 = A.z - (ok + A.x * oku + A.y * okv);
-  const float f = f0 * nd;
-  if (f < kEps || f > minT) return false;
-  const float hu = oku + f * dku;
-  const float hv = okv + f * dkv;
-  const float lambda = B.z + hu * B.x + hv * B.y;
-  if (lambda < 0.f) return false;
-  const float mue = C.z + hu * C.x + hv * C.y;
-  if (mue < 0.f || mue + lambda > 1.f) return false;
-  if (!(f > kEps && f < minT)) return false;
-  t = f;
-  u = lambda;
-  v = mue;
-  return true;
-}
-
-template <int PRIM>
-__device__ __forceinline__ bool prim_test(const float* tris, int slot, const Ray& r, float minT,
-                                          float& t, float& u, float& v) {
-  const float4* p = reinterpret_cast<const float4*>(tris + 12 * (size_t)slot);
-  const float4 A = p[0], B = p[1], C = p[2];
-  if (PRIM == 0) return ks_test(A, B, C, r, minT, t, u, v);
-  return wald_test(A, B, C, r, minT, t, u, v);
-}
-
-__device__ __forceinline__ void load_ray(const DevRays& rays, int64_t i, Ray& r) {
-  r.o[0] = rays.ox[i];
-  r.o[1] = rays.oy[i];
-  r.o[2] = rays.oz[i];
-  r.d[0] = rays.dx[i];
-  r.d[1] = rays.dy[i];
-  r.d[2] = rays.dz[i];
-  // RayPacket::actualComputeInverseDirections, scalar branch (RayPacket.cc:105-109)
-  r.inv[0] = 1.0f / r.d[0];
-  r.inv[1] = 1.0f / r.d[1];
-  r.inv[2] = 1.0f / r.d[2];
-}
-
-__device__ __forceinline__ void store_hit(const DevHits& h, int64_t i, int32_t prim, float t,
-                                          float u, float v, bool occ) {
-  if (h.prim) h.prim[i] = prim;
-  if (h.t) h.t[i] = t;
-  if (h.u) h.u[i] = u;
-  if (h.v) h.v[i] = v;
-  if (h.occluded) h.occluded[i] = occ ? 1 : 0;
-}
 
 // ---------------------------------------------------------------------------
 // Single-ray traversal
@@ -205,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_ray(DevScene sc, DevRays rays,
           one_first = ((M.z >> 2) & 1) == 0;
         } else {
           const int ax = M.z & 3;
-          const float da = ax == 0 ? r.d[0] : (ax == 1 ? r.d[1] : r.d[2]);
+          const float da = pick3(ax, r.d[0], r.d[1], r.d[2]);
           one_first = !(da > 0.f);
         }
         const int fref = one_first ? M.y : M.x;
@@ -445,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_packet(DevScene sc, DevRays ra
       if (ANY) {
         front = (nd.bits & 1) ? 0 : 1;
       } else {
-        const float da = nd.axis == 0 ? r.d[0] : (nd.axis == 1 ? r.d[1] : r.d[2]);
+        const float da = pick3(nd.axis, r.d[0], r.d[1], r.d[2]);
         front = (__shfl(da, fa, 64) > 0.f) ? 0 : 1;
       }
       stk[wv][sp++] = make_int2(nd.child + 1 - front, fa);

@@ -1,0 +1,143 @@
+// Device helpers shared by the traversal kernels: the ray record, the reference's slab
+// test (DynBVH::firstIntersects scalar loop, Model/Groups/DynBVH.cc:666-680), the
+// Kensler-Shirley and Wald tests (KenslerShirleyTriangle.cc:336-377,
+// WaldTriangle.cc:489-555) followed by RayPacket::hit (Interface/RayPacket.h:531-542),
+// and ray/hit SoA access. Everything is binary32 without contraction (the library is
+// compiled with -ffp-contract=off) and IEEE division.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mh_device.h"
+
+namespace mh {
+
+constexpr float kEps = 1.e-4f;   // T_EPSILON
+constexpr float kMaxT = 1.e19f;  // MAXT
+
+// a, b or c for k = 0, 1, 2, chosen with bit masks so that the compiler cannot turn the
+// choice into an indexed (scratch) access.
+static __device__ __forceinline__ float pick3(int k, float a, float b, float c) {
+  const unsigned m0 = 0u - (unsigned)(k == 0), m1 = 0u - (unsigned)(k == 1),
+                 m2 = 0u - (unsigned)(k == 2);
+  return __uint_as_float((__float_as_uint(a) & m0) | (__float_as_uint(b) & m1) |
+                         (__float_as_uint(c) & m2));
+}
+
+struct Ray {
+  float o[3], d[3], inv[3];
+};
+
+// Slab test of DynBVH::firstIntersects (scalar loop, DynBVH.cc:666-680).
+static __device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz,
+                                     const Ray& r, float minT, float& tnear) {
+  float tmin = kEps, tmax = minT;
+  float t0 = (lx - r.o[0]) * r.inv[0];
+  float t1 = (hx - r.o[0]) * r.inv[0];
+  float nr = (t0 < t1) ? t0 : t1;
+  float fr = (t0 < t1) ? t1 : t0;
+  tmin = (tmin < nr) ? nr : tmin;
+  tmax = (fr < tmax) ? fr : tmax;
+  t0 = (ly - r.o[1]) * r.inv[1];
+  t1 = (hy - r.o[1]) * r.inv[1];
+  nr = (t0 < t1) ? t0 : t1;
+  fr = (t0 < t1) ? t1 : t0;
+  tmin = (tmin < nr) ? nr : tmin;
+  tmax = (fr < tmax) ? fr : tmax;
+  t0 = (lz - r.o[2]) * r.inv[2];
+  t1 = (hz - r.o[2]) * r.inv[2];
+  nr = (t0 < t1) ? t0 : t1;
+  fr = (t0 < t1) ? t1 : t0;
+  tmin = (tmin < nr) ? nr : tmin;
+  tmax = (fr < tmax) ? fr : tmax;
+  tnear = tmin;
+  return tmin <= tmax;
+}
+
+// KenslerShirleyTriangle::intersect, scalar branch (KenslerShirleyTriangle.cc:336-377),
+// then RayPacket::hit (Interface/RayPacket.h:531-542). Slot data: p0, e0, e1, n.
+static __device__ __forceinline__ bool ks_test(const float4 A, const float4 B, const float4 C,
+                                        const Ray& r, float minT, float& t, float& u, float& v) {
+  const float n0 = C.y, n1 = C.z, n2 = C.w;
+  const float det = n0 * r.d[0] + n1 * r.d[1] + n2 * r.d[2];
+  const float rcp = 1.0f / det;
+  const float g0 = A.x - r.o[0], g1 = A.y - r.o[1], g2 = A.z - r.o[2];
+  const float tt = (n0 * g0 + n1 * g1 + n2 * g2) * rcp;
+  if (tt > minT - kEps || tt < kEps) return false;
+  const float i0 = g1 * r.d[2] - g2 * r.d[1];
+  const float i1 = g2 * r.d[0] - g0 * r.d[2];
+  const float i2 = g0 * r.d[1] - g1 * r.d[0];
+  const float uu = (i0 * B.z + i1 * B.w + i2 * C.x) * rcp;
+  if (uu < 0.0f) return false;
+  const float vv = (i0 * A.w + i1 * B.x + i2 * B.y) * rcp;
+  if (uu + vv > 1.0f || vv < 0.0f) return false;
+  if (!(tt > kEps && tt < minT)) return false;
+  t = tt;
+  u = uu;
+  v = vv;
+  return true;
+}
+
+// WaldTriangle::intersect, scalar branch (WaldTriangle.cc:489-555), then RayPacket::hit.
+static __device__ __forceinline__ bool wald_test(const float4 A, const float4 B, const float4 C,
+                                          const Ray& r, float minT, float& t, float& u,
+                                          float& v) {
+  // Projection axis k and its successors ku = (k+1)%3, kv = (k+2)%3.
+  const int k = __float_as_int(A.w);
+  const float dk = pick3(k, r.d[0], r.d[1], r.d[2]);
+  const float dku = pick3(k, r.d[1], r.d[2], r.d[0]);
+  const float dkv = pick3(k, r.d[2], r.d[0], r.d[1]);
+  const float ok = pick3(k, r.o[0], r.o[1], r.o[2]);
+  const float oku = pick3(k, r.o[1], r.o[2], r.o[0]);
+  const float okv = pick3(k, r.o[2], r.o[0], r.o[1]);
+  const float nd0 = A.x * dku + A.y * dkv + dk;
+  const float nd = 1.f / nd0;
+  const float f0 = A.z - (ok + A.x * oku + A.y * okv);
+  const float f = f0 * nd;
+  if (f < kEps || f > minT) return false;
+  const float hu = oku + f * dku;
+  const float hv = okv + f * dkv;
+  const float lambda = B.z + hu * B.x + hv * B.y;
+  if (lambda < 0.f) return false;
+  const float mue = C.z + hu * C.x + hv * C.y;
+  if (mue < 0.f || mue + lambda > 1.f) return false;
+  if (!(f > kEps && f < minT)) return false;
+  t = f;
+  u = lambda;
+  v = mue;
+  return true;
+}
+
+template <int PRIM>
+static __device__ __forceinline__ bool prim_test(const float* tris, int slot, const Ray& r, float minT,
+                                          float& t, float& u, float& v) {
+  const float4* p = reinterpret_cast<const float4*>(tris + 12 * (size_t)slot);
+  const float4 A = p[0], B = p[1], C = p[2];
+  if (PRIM == 0) return ks_test(A, B, C, r, minT, t, u, v);
+  return wald_test(A, B, C, r, minT, t, u, v);
+}
+
+static __device__ __forceinline__ void load_ray(const DevRays& rays, int64_t i, Ray& r) {
+  r.o[0] = rays.ox[i];
+  r.o[1] = rays.oy[i];
+  r.o[2] = rays.oz[i];
+  r.d[0] = rays.dx[i];
+  r.d[1] = rays.dy[i];
+  r.d[2] = rays.dz[i];
+  // RayPacket::actualComputeInverseDirections, scalar branch (RayPacket.cc:105-109)
+  r.inv[0] = 1.0f / r.d[0];
+  r.inv[1] = 1.0f / r.d[1];
+  r.inv[2] = 1.0f / r.d[2];
+}
+
+static __device__ __forceinline__ void store_hit(const DevHits& h, int64_t i, int32_t prim, float t,
+                                          float u, float v, bool occ) {
+  if (h.prim) h.prim[i] = prim;
+  if (h.t) h.t[i] = t;
+  if (h.u) h.u[i] = u;
+  if (h.v) h.v[i] = v;
+  if (h.occluded) h.occluded[i] = occ ? 1 : 0;
+}
+
+
+}  // namespace mh

@@ -261,3 +261,87 @@ def config1_camera(xres: int = 1920, yres: int = 1080) -> dict:
 
 
 CONFIG1_LIGHT = (2.0, 2.0, -2.0)
+
+
+def surface_points(verts, idx, n: int, seed: int, stream0: int):
+    """n uniformly chosen (triangle, barycentric) points; returns (P, tri, face-up normal)."""
+    T = len(idx)
+    r = pcg32(seed, np.arange(stream0, stream0 + n, dtype=np.uint64), 3)
+    tri = (u01(r[0]) * T).astype(np.int64).clip(0, T - 1)
+    a, b = u01(r[1]), u01(r[2])
+    flip = a + b > 1
+    a = np.where(flip, 1 - a, a)
+    b = np.where(flip, 1 - b, b)
+    p = verts[idx[tri]].astype(np.float64)
+    P = p[:, 0] * (1 - a - b)[:, None] + p[:, 1] * a[:, None] + p[:, 2] * b[:, None]
+    nrm = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    nrm = np.where((nrm[:, 1] < 0)[:, None], -nrm, nrm)  # heightfield: face the sky
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    return P, tri, nrm
+
+
+def _soa(o, d, tmax):
+    o = o.astype(np.float32)
+    d = d.astype(np.float32)
+    return {"ox": np.ascontiguousarray(o[:, 0]), "oy": np.ascontiguousarray(o[:, 1]),
+            "oz": np.ascontiguousarray(o[:, 2]), "dx": np.ascontiguousarray(d[:, 0]),
+            "dy": np.ascontiguousarray(d[:, 1]), "dz": np.ascontiguousarray(d[:, 2]),
+            "tmax": np.full(len(o), tmax, np.float32)}
+
+
+def batch_rays(kind: str, n: int, verts, idx, seed: int, stream0: int, light=CONFIG1_LIGHT,
+               eye=(0.0, 1.6, -0.6), lookat=(0.0, 0.0, 0.0)) -> dict:
+    """One ray batch of the batching workload.
+
+    primary: pinhole rays from `eye` through uniform image positions (the camera of the
+    1M-triangle frame); shadow: from uniform surface points toward the light, unnormalised,
+    minT = 1 - T_EPSILON; bounce: cosine-weighted about the surface normal, minT = MAXT.
+    """
+    if kind == "primary":
+        r = pcg32(seed, np.arange(stream0, stream0 + n, dtype=np.uint64), 2)
+        x = 2 * u01(r[0]) - 1
+        y = 2 * u01(r[1]) - 1
+        e = np.asarray(eye, np.float64)
+        direction = np.asarray(lookat, np.float64) - e
+        nz = np.linalg.norm(direction)
+        v = np.cross(direction, [0.0, 1.0, 0.0])
+        v /= np.linalg.norm(v)
+        u = np.cross(v, direction)
+        u /= np.linalg.norm(u)
+        vf = 36.0
+        hf = np.degrees(2 * np.arctan(np.tan(np.radians(vf / 2)) * 16 / 9))
+        u *= nz * np.tan(np.radians(vf / 2))
+        v *= nz * np.tan(np.radians(hf / 2))
+        d = v[None] * x[:, None] + u[None] * y[:, None] + direction[None]
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        return _soa(np.broadcast_to(e, d.shape), d, MAXT)
+    P, tri, nrm = surface_points(verts, idx, n, seed, stream0)
+    if kind == "shadow":
+        d = np.asarray(light, np.float64)[None] - P
+        return _soa(P, d, np.float32(1) - T_EPSILON)
+    r = pcg32(seed, np.arange(stream0, stream0 + n, dtype=np.uint64) + np.uint64(1 << 40), 2)
+    u1, u2 = u01(r[0]), u01(r[1])
+    rad = np.sqrt(u1)
+    lx, ly, lz = rad * np.cos(2 * np.pi * u2), rad * np.sin(2 * np.pi * u2), np.sqrt(1 - u1)
+    a = np.where((np.abs(nrm[:, 0]) > 0.9)[:, None], [[0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0]])
+    t1 = np.cross(nrm, a)
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(nrm, t1)
+    d = t1 * lx[:, None] + t2 * ly[:, None] + nrm * lz[:, None]
+    return _soa(P, d, MAXT)
+
+
+def config2_batches(verts, idx, n_batches: int = 64, lo: int = 1_000, hi: int = 2_000_000,
+                    seed: int = 1):
+    """Batching workload: sizes log-uniform in [lo, hi], types 40% primary, 30% shadow
+    (any-hit), 30% diffuse bounce. Returns [(kind, rays)]."""
+    r = pcg32(seed, np.arange(n_batches, dtype=np.uint64) + np.uint64(1 << 50), 2)
+    sizes = np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * u01(r[0])).astype(np.int64)
+    tu = u01(r[1])
+    kinds = np.where(tu < 0.4, "primary", np.where(tu < 0.7, "shadow", "bounce"))
+    out = []
+    start = 0
+    for b in range(n_batches):
+        out.append((str(kinds[b]), batch_rays(str(kinds[b]), int(sizes[b]), verts, idx, seed, start)))
+        start += int(sizes[b])
+    return out
