@@ -192,7 +192,13 @@ typedef struct mh_launch_config {
   int stack_lds;     /* per-lane LDS stack ring: 8, 16 or 32 entries */
   int block;         /* lanes per workgroup: 64, 128 or 256 */
   int blocks_per_cu; /* persistent workgroups per compute unit */
-  int refill;        /* idle lanes of a wave that trigger a refill, 1..64 */
+  int refill;        /* idle lanes of a wave that trigger a refill, 1..63; 64 = whole-wave
+                        chunks (a wave traces `chunk` consecutive rays 64 at a time) */
+  int chunk;         /* rays a wave takes per grab from the shared counter, a multiple of 64
+                        (its first grab is assigned statically); 0 = static stride over the
+                        rays with no counter (whole-wave mode only) */
+  int xcd_remap;     /* 1 = give each XCD (every 8th workgroup) a contiguous share of the
+                        rays so that neighbouring rays share an L2 */
 } mh_launch_config;
 
 typedef struct mh_frame mh_frame;
@@ -209,12 +215,104 @@ int mh_frame_capture(mh_frame* frame, void* stream);
 int mh_frame_replay(mh_frame* frame, void* stream);
 /* Average milliseconds per launch over `iters` launches (graph replays when use_graph). */
 int mh_frame_time(mh_frame* frame, int iters, int use_graph, void* stream, float* ms);
-/* Sweeps layout x stack x block x blocks_per_cu x refill, keeps the fastest and stores it
+/* Sweeps layout x stack x block x blocks_per_cu x chunk x xcd_remap x refill, keeps the
+ * fastest and stores it
  * in `cache_path` (JSON, may be NULL) keyed by the scene and batch mix; a cached entry
  * is reused without sweeping unless `force`. */
 int mh_frame_autotune(mh_frame* frame, int iters, const char* cache_path, int force,
                       mh_launch_config* best, float* best_ms);
 int mh_frame_destroy(mh_frame* frame);
+
+/* ---- Ray generation on the device -----------------------------------------------
+ * Writable SoA ray arrays (masked may be NULL for camera rays). */
+typedef struct mh_ray_buffers {
+  float* ox;
+  float* oy;
+  float* oz;
+  float* dx;
+  float* dy;
+  float* dz;
+  float* tmax;
+  uint8_t* masked;
+} mh_ray_buffers;
+
+/* Pinhole camera, PinholeCamera::setup/makeRays (Model/Cameras/PinholeCamera.cc:167-230).
+ * hfov_deg 0 derives the horizontal field of view from the aspect ratio. */
+typedef struct mh_camera {
+  float eye[3];
+  float lookat[3];
+  float up[3];
+  float vfov_deg;
+  float hfov_deg;
+  int xres, yres; /* multiples of 8 */
+  int spp;
+  uint64_t seed;
+} mh_camera;
+
+/* xres*yres*spp normalised camera rays: sample-major, pixels in 8x8 tiles, row-major
+ * inside a tile (one 64-ray packet per tile and sample). */
+int mh_gen_camera_rays(const mh_camera* cam, const mh_ray_buffers* out, void* stream);
+/* One shadow ray per primary ray toward `light` (PointLight::computeLight +
+ * HardShadows, Model/Lights/PointLight.cc:31-75, Engine/Shadows/HardShadows.cc:87-124);
+ * misses and back-facing rays are masked. Reads the scene's current vertices. */
+int mh_gen_shadow_rays(const mh_scene* scene, const mh_rays* primary, const mh_hits* hits,
+                       int64_t n, const float light[3], const mh_ray_buffers* out, void* stream);
+/* One cosine-weighted diffuse bounce per primary ray; misses are masked. */
+int mh_gen_bounce_rays(const mh_scene* scene, const mh_rays* primary, const mh_hits* hits,
+                       int64_t n, uint64_t seed, const mh_ray_buffers* out, void* stream);
+
+/* ---- Serving loop -----------------------------------------------------------------
+ * No reference counterpart. A server keeps scenes resident and renders requests (a
+ * camera over one of its scenes: closest-hit primary rays, optionally one shadow ray
+ * per hit) with continuous batching: each round of its scheduler thread admits waiting
+ * requests while their device buffers fit the HBM budget, then launches one persistent
+ * batch trace per scene over a chunk of every admitted request's remaining rays (scenes
+ * on separate streams, overlapping), and completes requests whose rays are done by
+ * copying their hits to the host. Per-ray results equal mh_trace with packet_size 1. */
+typedef struct mh_server mh_server;
+
+typedef struct mh_server_opts {
+  size_t hbm_budget;   /* bytes of device request buffers admitted at once [1 GiB] */
+  int64_t chunk_rays;  /* rays of one request traced per round [1<<20] */
+  int max_active;      /* requests in flight [64] */
+  int shadows;         /* 1 = one shadow ray per primary hit toward `light` [0] */
+  float light[3];
+  mh_launch_config launch; /* zero = default configuration */
+} mh_server_opts;
+
+typedef struct mh_request {
+  int scene; /* index in the server's scene list */
+  mh_camera camera;
+} mh_request;
+
+typedef struct mh_request_info {
+  uint64_t id;
+  int status;        /* MH_OK or the error that ended the request */
+  int64_t rays;      /* rays traced (primary + shadow) */
+  double submit_ms;  /* server clock */
+  double admit_ms;
+  double done_ms;    /* hits on the host */
+} mh_request_info;
+
+typedef struct mh_server_stats {
+  uint64_t submitted, completed, rejected;
+  int64_t rays;        /* rays traced since creation */
+  double busy_ms;      /* wall time with at least one request in flight */
+  int active, pending;
+  size_t hbm_in_use;
+} mh_server_stats;
+
+int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_server_opts* opts,
+                     mh_server** out);
+/* Queues a request; returns at once. Requests larger than the budget are rejected. */
+int mh_server_submit(mh_server* server, const mh_request* req, uint64_t* id);
+/* Waits for a request; copies its hits (any pointer may be NULL): prim and t per primary
+ * ray (xres*yres*spp, the camera ray order), occluded per primary ray (shadows). */
+int mh_server_wait(mh_server* server, uint64_t id, int32_t* prim, float* t, uint8_t* occluded,
+                   mh_request_info* info);
+int mh_server_get_stats(mh_server* server, mh_server_stats* stats);
+double mh_server_clock_ms(const mh_server* server);
+int mh_server_destroy(mh_server* server);
 
 /* Device memory helpers for callers without their own allocator. */
 int mh_malloc(void** ptr, size_t bytes);

@@ -56,18 +56,54 @@ class Batch(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("layout", C.c_int), ("stack_lds", C.c_int), ("block", C.c_int),
-                ("blocks_per_cu", C.c_int), ("refill", C.c_int)]
+                ("blocks_per_cu", C.c_int), ("refill", C.c_int), ("chunk", C.c_int),
+                ("xcd_remap", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "LaunchConfig":
+        return cls(d["layout"], d["stack_lds"], d["block"], d["blocks_per_cu"], d["refill"],
+                   d.get("chunk", 64), d.get("xcd_remap", 0))
 
 
 MH_LAYOUT_PAIR = 0
 MH_LAYOUT_REF = 1
 
 
+class RayBuffers(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax", "masked")]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("lookat", C.c_float * 3), ("up", C.c_float * 3),
+                ("vfov_deg", C.c_float), ("hfov_deg", C.c_float), ("xres", C.c_int),
+                ("yres", C.c_int), ("spp", C.c_int), ("seed", C.c_uint64)]
+
+
 class BuildOpts(C.Structure):
     _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("chunk_rays", C.c_int64), ("max_active", C.c_int),
+                ("shadows", C.c_int), ("light", C.c_float * 3), ("launch", LaunchConfig)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int), ("camera", Camera)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("rays", C.c_int64),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("rays", C.c_int64), ("busy_ms", C.c_double), ("active", C.c_int),
+                ("pending", C.c_int), ("hbm_in_use", C.c_size_t)]
 
 
 # Every symbol include/mh.h declares, with its ctypes signature.
@@ -108,6 +144,18 @@ _SIGS = {
     "mh_frame_autotune": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(LaunchConfig),
                                     C.POINTER(C.c_float)]),
     "mh_frame_destroy": (C.c_int, [_P]),
+    "mh_gen_camera_rays": (C.c_int, [C.POINTER(Camera), C.POINTER(RayBuffers), _P]),
+    "mh_gen_shadow_rays": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64,
+                                     C.POINTER(C.c_float * 3), C.POINTER(RayBuffers), _P]),
+    "mh_gen_bounce_rays": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint64,
+                                     C.POINTER(RayBuffers), _P]),
+    "mh_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(ServerOpts),
+                                   C.POINTER(C.c_void_p)]),
+    "mh_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "mh_server_wait": (C.c_int, [_P, C.c_uint64, _P, _P, _P, C.POINTER(RequestInfo)]),
+    "mh_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "mh_server_clock_ms": (C.c_double, [_P]),
+    "mh_server_destroy": (C.c_int, [_P]),
     "mh_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "mh_free": (C.c_int, [_P]),
     "mh_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),

@@ -73,12 +73,32 @@ class DeviceRays:
             arrs["masked"] = DeviceArray.from_numpy(np.asarray(rays["masked"], np.uint8))
         return cls(n, arrs)
 
+    @classmethod
+    def alloc(cls, n: int, masked: bool = True) -> "DeviceRays":
+        arrs = {k: DeviceArray(4 * n) for k in RAY_FIELDS}
+        if masked:
+            arrs["masked"] = DeviceArray(n)
+        return cls(n, arrs)
+
     def struct(self) -> L.Rays:
         r = L.Rays()
         for k in RAY_FIELDS:
             setattr(r, k, self.arrays[k].ptr)
         r.masked = self.arrays["masked"].ptr if "masked" in self.arrays else None
         return r
+
+    def buffers(self) -> L.RayBuffers:
+        b = L.RayBuffers()
+        for k in RAY_FIELDS:
+            setattr(b, k, self.arrays[k].ptr)
+        b.masked = self.arrays["masked"].ptr if "masked" in self.arrays else None
+        return b
+
+    def download(self) -> dict:
+        out = {k: self.arrays[k].download(np.float32, self.n) for k in RAY_FIELDS}
+        if "masked" in self.arrays:
+            out["masked"] = self.arrays["masked"].download(np.uint8, self.n)
+        return out
 
 
 @dataclass
@@ -270,8 +290,7 @@ class Frame:
     def set_config(self, cfg: dict | None) -> None:
         c = None
         if cfg is not None:
-            c = L.LaunchConfig(cfg["layout"], cfg["stack_lds"], cfg["block"],
-                               cfg["blocks_per_cu"], cfg["refill"])
+            c = L.LaunchConfig.from_dict(cfg)
         L.check(L.load().mh_frame_set_config(self.h, C.byref(c) if c is not None else None))
 
     def config(self) -> dict:
@@ -303,6 +322,98 @@ class Frame:
     def close(self) -> None:
         if getattr(self, "h", None):
             L.load().mh_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def camera(eye, lookat, up, vfov_deg, xres, yres, spp=1, seed=0, hfov_deg=0.0) -> L.Camera:
+    c = L.Camera()
+    c.eye[:] = [float(x) for x in eye]
+    c.lookat[:] = [float(x) for x in lookat]
+    c.up[:] = [float(x) for x in up]
+    c.vfov_deg, c.hfov_deg = vfov_deg, hfov_deg
+    c.xres, c.yres, c.spp, c.seed = xres, yres, spp, seed
+    return c
+
+
+def gen_camera_rays(cam: L.Camera, out: DeviceRays, stream=None) -> None:
+    assert out.n == cam.xres * cam.yres * cam.spp
+    b = out.buffers()
+    L.check(L.load().mh_gen_camera_rays(C.byref(cam), C.byref(b), stream))
+
+
+def gen_shadow_rays(scene: "Scene", primary: DeviceRays, hits: DeviceHits, light, out: DeviceRays,
+                    stream=None) -> None:
+    lt = (C.c_float * 3)(*[float(x) for x in light])
+    r, h, b = primary.struct(), hits.struct(), out.buffers()
+    L.check(L.load().mh_gen_shadow_rays(scene.h, C.byref(r), C.byref(h), primary.n, C.byref(lt),
+                                        C.byref(b), stream))
+
+
+def gen_bounce_rays(scene: "Scene", primary: DeviceRays, hits: DeviceHits, seed: int,
+                    out: DeviceRays, stream=None) -> None:
+    r, h, b = primary.struct(), hits.struct(), out.buffers()
+    L.check(L.load().mh_gen_bounce_rays(scene.h, C.byref(r), C.byref(h), primary.n, seed,
+                                        C.byref(b), stream))
+
+
+class Server:
+    """Serving loop over resident scenes (mh_server_*): submit camera requests, wait for hits."""
+
+    def __init__(self, scenes: list, hbm_budget: int = 0, chunk_rays: int = 0,
+                 max_active: int = 0, shadows: bool = False, light=(0.0, 0.0, 0.0),
+                 launch: dict | None = None):
+        self.scenes = list(scenes)
+        o = L.ServerOpts()
+        o.hbm_budget, o.chunk_rays, o.max_active, o.shadows = hbm_budget, chunk_rays, max_active, int(shadows)
+        o.light[:] = [float(x) for x in light]
+        if launch is not None:
+            o.launch = L.LaunchConfig.from_dict(launch)
+        self.shadows = bool(shadows)
+        arr = (C.c_void_p * len(self.scenes))(*[sc.h for sc in self.scenes])
+        h = C.c_void_p()
+        L.check(L.load().mh_server_create(arr, len(self.scenes), C.byref(o), C.byref(h)))
+        self.h = h.value
+        self._sizes = {}
+
+    def submit(self, scene: int, cam: L.Camera) -> int:
+        r = L.Request(scene, cam)
+        rid = C.c_uint64()
+        L.check(L.load().mh_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._sizes[rid.value] = cam.xres * cam.yres * cam.spp
+        return rid.value
+
+    def wait(self, rid: int) -> dict:
+        n = self._sizes.pop(rid, 0)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        occ = np.empty(n, np.uint8) if self.shadows else None
+        info = L.RequestInfo()
+        L.check(L.load().mh_server_wait(self.h, rid, prim.ctypes.data, t.ctypes.data,
+                                        occ.ctypes.data if occ is not None else None,
+                                        C.byref(info)))
+        out = {"prim": prim, "t": t, "id": info.id, "status": info.status, "rays": info.rays,
+               "submit_ms": info.submit_ms, "admit_ms": info.admit_ms, "done_ms": info.done_ms}
+        if occ is not None:
+            out["occluded"] = occ
+        return out
+
+    def stats(self) -> dict:
+        st = L.ServerStats()
+        L.check(L.load().mh_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in L.ServerStats._fields_}
+
+    def clock_ms(self) -> float:
+        return L.load().mh_server_clock_ms(self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.load().mh_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -42,11 +42,9 @@ struct mh_frame {
   ~mh_frame() { drop_graph(); }
 };
 
-namespace {
+namespace mh_internal {
 
-const mh_launch_config kDefault = {MH_LAYOUT_PAIR, 8, 256, 8, 16};
-
-int check_config(const mh_launch_config& c) {
+int check_launch_config(const mh_launch_config& c) {
   if (c.layout != MH_LAYOUT_PAIR && c.layout != MH_LAYOUT_REF)
     return fail(MH_E_INVALID, "layout %d", c.layout);
   if (c.stack_lds != 8 && c.stack_lds != 16 && c.stack_lds != 32)
@@ -56,10 +54,22 @@ int check_config(const mh_launch_config& c) {
   if (c.blocks_per_cu < 1 || c.blocks_per_cu > 64)
     return fail(MH_E_INVALID, "blocks_per_cu %d", c.blocks_per_cu);
   if (c.refill < 1 || c.refill > 64) return fail(MH_E_INVALID, "refill %d not in 1..64", c.refill);
+  if (c.chunk < 0 || c.chunk > (1 << 20) || c.chunk % 64 != 0)
+    return fail(MH_E_INVALID, "chunk %d is not a multiple of 64 in 0..2^20", c.chunk);
+  if (c.chunk == 0 && c.refill < 64)
+    return fail(MH_E_INVALID, "chunk 0 (static stride) needs refill 64");
+  if (c.xcd_remap != 0 && c.xcd_remap != 1) return fail(MH_E_INVALID, "xcd_remap %d", c.xcd_remap);
   const size_t lds = (size_t)c.stack_lds * c.block * 8 + 256 * 8;
   if (lds > 65536) return fail(MH_E_UNSUPPORTED, "LDS per workgroup %zu > 64 KiB", lds);
   return MH_OK;
 }
+
+}  // namespace mh_internal
+
+namespace {
+
+const mh_launch_config kDefault = {MH_LAYOUT_PAIR, 8, 256, 16, 64, 256, 1};
+
 
 int build_table(const mh_batch* b, int n, std::vector<uint8_t>& out, int64_t& total, int& n_any) {
   if (!b || n < 1 || n > MH_MAX_BATCHES)
@@ -98,7 +108,7 @@ int build_table(const mh_batch* b, int n, std::vector<uint8_t>& out, int64_t& to
 }
 
 int prepare(mh_frame* f, const mh_launch_config& c) {
-  int rc = check_config(c);
+  int rc = mh_internal::check_launch_config(c);
   if (rc) return rc;
   const int blocks = c.blocks_per_cu * mh_internal::cu_count();
   const int64_t stride = (int64_t)blocks * c.block;
@@ -108,7 +118,8 @@ int prepare(mh_frame* f, const mh_launch_config& c) {
     if ((rc = f->spill.reserve((size_t)need * stride * 8))) return rc;
   }
   if (c.layout != f->cfg.layout || c.stack_lds != f->cfg.stack_lds || c.block != f->cfg.block ||
-      c.blocks_per_cu != f->cfg.blocks_per_cu || c.refill != f->cfg.refill)
+      c.blocks_per_cu != f->cfg.blocks_per_cu || c.refill != f->cfg.refill ||
+      c.chunk != f->cfg.chunk || c.xcd_remap != f->cfg.xcd_remap)
     f->drop_graph();
   f->cfg = c;
   f->blocks = blocks;
@@ -129,6 +140,8 @@ int launch(mh_frame* f, void* stream) {
   bl.stack_lds = f->cfg.stack_lds;
   bl.layout = f->cfg.layout;
   bl.refill = f->cfg.refill;
+  bl.chunk = f->cfg.chunk;
+  bl.xcd_remap = f->cfg.xcd_remap;
   HIP_OK((hipError_t)mh::launch_trace_batch(f->scene->dev(), bl, stream));
   return MH_OK;
 }
@@ -327,16 +340,18 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
   if (cache_path) cache = read_cache(cache_path);
   if (cache_path && !force && cache.count(key)) {
     const std::string& obj = cache[key];
-    double v[6];
-    const char* names[6] = {"layout", "stack_lds", "block", "blocks_per_cu", "refill", "ms"};
+    double v[8];
+    const char* names[8] = {"layout", "stack_lds", "block", "blocks_per_cu",
+                            "refill", "chunk", "xcd_remap", "ms"};
     bool ok = true;
-    for (int i = 0; i < 6; ++i) ok = ok && field(obj, names[i], v[i]);
+    for (int i = 0; i < 8; ++i) ok = ok && field(obj, names[i], v[i]);
     if (ok) {
-      mh_launch_config c{(int)v[0], (int)v[1], (int)v[2], (int)v[3], (int)v[4]};
-      if (check_config(c) == MH_OK) {
+      mh_launch_config c{(int)v[0], (int)v[1], (int)v[2], (int)v[3],
+                         (int)v[4], (int)v[5], (int)v[6]};
+      if (mh_internal::check_launch_config(c) == MH_OK) {
         if ((rc = prepare(f, c))) return rc;
         if (best_out) *best_out = c;
-        if (best_ms_out) *best_ms_out = (float)v[5];
+        if (best_ms_out) *best_ms_out = (float)v[7];
         return MH_OK;
       }
     }
@@ -347,8 +362,9 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
   const int stacks[3] = {8, 16, 32};
   const int blocks[3] = {64, 128, 256};
   const int bpcs[6] = {2, 4, 8, 12, 16, 24};
+  // Stage 1: layout x stack x block x workgroups per CU, whole-wave chunks of 256 rays.
   auto trial = [&](const mh_launch_config& c) -> int {
-    if (check_config(c) != MH_OK) return MH_OK;  // outside the LDS budget: skip
+    if (mh_internal::check_launch_config(c) != MH_OK) return MH_OK;  // outside the LDS budget: skip
     int r = prepare(f, c);
     if (r) return r;
     if ((r = launch(f, nullptr))) return r;  // warm-up
@@ -364,22 +380,35 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
     for (int s : stacks)
       for (int b : blocks)
         for (int p : bpcs)
-          if ((rc = trial(mh_launch_config{l, s, b, p, 16}))) return rc;
-  const int refills[4] = {4, 8, 32, 48};
-  const mh_launch_config base = best;
-  for (int r : refills) {
-    mh_launch_config c = base;
-    c.refill = r;
-    if ((rc = trial(c))) return rc;
-  }
+          if ((rc = trial(mh_launch_config{l, s, b, p, 64, 256, 1}))) return rc;
+  // Stage 2: schedule at the best shape: chunk size (0 = static stride) x XCD remap.
+  const int chunks[6] = {0, 64, 128, 256, 512, 1024};
+  mh_launch_config base = best;
+  for (int c : chunks)
+    for (int x = 0; x < 2; ++x) {
+      mh_launch_config t = base;
+      t.chunk = c;
+      t.xcd_remap = x;
+      if ((rc = trial(t))) return rc;
+    }
+  // Stage 3: per-lane refill thresholds.
+  const int refills[5] = {4, 8, 16, 32, 48};
+  base = best;
+  for (int r : refills)
+    for (int c : {64, 256}) {
+      mh_launch_config t = base;
+      t.refill = r;
+      t.chunk = c;
+      if ((rc = trial(t))) return rc;
+    }
   if ((rc = prepare(f, best))) return rc;
   if (cache_path) {
     char buf[256];
     std::snprintf(buf, sizeof(buf),
                   "{\"layout\": %d, \"stack_lds\": %d, \"block\": %d, \"blocks_per_cu\": %d, "
-                  "\"refill\": %d, \"ms\": %.5f}",
+                  "\"refill\": %d, \"chunk\": %d, \"xcd_remap\": %d, \"ms\": %.5f}",
                   best.layout, best.stack_lds, best.block, best.blocks_per_cu, best.refill,
-                  best_ms);
+                  best.chunk, best.xcd_remap, best_ms);
     cache[key] = buf;
     write_cache(cache_path, cache);
   }

@@ -24,7 +24,13 @@ int ensure_init();
 int cu_count();
 const char* device_name();
 
-#define HIP_OK(call)                                                                              \
+// Validates a batch launch configuration (frame_api.cpp); MH_OK or an error code.
+int check_launch_config(const mh_launch_config& c);
+
+// PinholeCamera::setup for the device generator (rays_api.cpp).
+int camera_setup(const mh_camera& c, mh::CameraDev& d);
+
+#define HIP_OK(call)                                                                            \
   do {                                                                                            \
     hipError_t e_ = (call);                                                                       \
     if (e_ != hipSuccess)                                                                         \

@@ -99,7 +99,9 @@ struct BatchLaunch {
   int block;      // 64, 128 or 256 lanes
   int stack_lds;  // 8, 16 or 32
   int layout;     // kLayoutPair or kLayoutRef
-  int refill;     // idle lanes that trigger a refill, 1..64
+  int refill;     // idle lanes that trigger a refill, 1..63; 64 = whole-wave chunks
+  int chunk;      // rays per counter grab (multiple of 64); 0 = static stride, no counter
+  int xcd_remap;  // contiguous ray shares per XCD
 };
 
 int launch_trace_batch(const DevScene& sc, const BatchLaunch& bl, void* stream);
@@ -129,6 +131,30 @@ int launch_prim_setup(const DevRefit& r, void* stream);  // tris from verts (all
 int launch_refit(const DevRefit& r, void* stream);       // tris + bounds, bottom-up
 int launch_displace_sine(float* verts, const float* base, int64_t n_verts, float amp, float freq,
                          float phase, void* stream);
+
+// genrays.hip: camera rays and secondary rays from hit records.
+struct RayOut {
+  float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+  uint8_t* masked;  // may be null for camera rays
+};
+struct CameraDev {
+  float eye[3], U[3], V[3], W[3];  // up * tan(vfov/2) * |W|, right * tan(hfov/2) * |W|, lookat-eye
+  int xres, yres, spp;             // xres, yres multiples of 8
+  unsigned long long seed;
+};
+struct SecondaryArgs {
+  DevRays in;            // the primary rays
+  const int32_t* hit_prim;
+  const float* hit_t;
+  const float* verts;
+  const uint32_t* idx;
+  float light[3];
+  unsigned long long seed;
+  int64_t n;
+  RayOut out;
+};
+int launch_camera(const CameraDev& c, const RayOut& out, void* stream);
+int launch_secondary(const SecondaryArgs& a, int kind /*0 shadow, 1 bounce*/, void* stream);
 
 // util.hip: streaming copy with 16-byte vector accesses (plus a byte tail).
 int launch_copy(void* dst, const void* src, size_t bytes, int cu_count, void* stream);

@@ -1,14 +1,8 @@
 // Traversal kernels for gfx950 (wave64).
 //
 // k_trace_ray: one ray per lane, DynBVH::intersect on a one-ray packet
-//   (Model/Groups/DynBVH.cc:101-404). Inner nodes are 64-byte PairNodes that hold both
-//   child boxes, so one fetch tests both children; the child the reference visits first
-//   (direction sign on the split axis for closest-hit, isLeftCheaper for any-hit,
-//   DynBVH.cc:386-398) is entered and the other is pushed with its entry distance. A
-//   popped entry is entered only if that distance is still <= the ray's minT, which is
-//   exactly the reference's later slab test of that child with the updated minT. The
-//   traversal stack is a per-lane ring of `S` entries in LDS ([slot][lane], conflict free)
-//   whose oldest entries spill to a per-thread global area when the ring is full.
+//   (Model/Groups/DynBVH.cc:101-404) over 64-byte PairNodes with an LDS ring stack; the
+//   traversal itself is traverse.h's, shared with the batch kernel.
 //
 // k_trace_packet: one RayPacket of up to 64 consecutive rays per wave, the reference's
 //   packet recursion (intersectNode/firstIntersects/lastIntersects, DynBVH.cc:274-905,
@@ -27,6 +21,7 @@
 
 #include "mh_device.h"
 #include "trace_common.h"
+#include "traverse.h"
 
 namespace mh {
 namespace {
@@ -36,128 +31,25 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kPacketStack = 128;
 
 // ---------------------------------------------------------------------------
-// Single-ray traversal
+// Single-ray traversal (traverse.h), grid-stride over the rays
 // ---------------------------------------------------------------------------
 template <bool ANY, int PRIM, int S, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_trace_ray(DevScene sc, DevRays rays, DevHits hits,
                                                       int64_t n, int2* spill,
                                                       int64_t spill_stride,
                                                       unsigned long long* counters) {
-  static_assert((S & (S - 1)) == 0, "ring size is a power of two");
-  __shared__ int s_ref[S * kBlock];
-  __shared__ float s_tn[S * kBlock];
+  __shared__ unsigned long long s_ring[S * kBlock];
   const int tid = threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   unsigned long long nbox = 0, ntri = 0;
   const RefNode root = sc.refs[0];
-
-  for (int64_t ri = gtid; ri < n; ri += stride) {
-    if (rays.masked && rays.masked[ri]) {
-      store_hit(hits, ri, -2, -kMaxT, 0.f, 0.f, true);
-      continue;
-    }
-    Ray r;
-    load_ray(rays, ri, r);
-    float minT = rays.tmax[ri];
-    int best = -1;
-    float bu = 0.f, bv = 0.f;
-    float tn;
-    bool go = slab(root.box.lo[0], root.box.hi[0], root.box.lo[1], root.box.hi[1],
-                   root.box.lo[2], root.box.hi[2], r, minT, tn);
-    if (STATS) ++nbox;
-    int cur = sc.root_ref;
-    int sp = 0, base = 0;
-    while (go) {
-      if (cur >= 0) {
-        const float4* np = reinterpret_cast<const float4*>(sc.pairs + cur);
-        const float4 A = np[0], B = np[1], C = np[2];
-        const int4 M = reinterpret_cast<const int4*>(np)[3];
-        float tn0, tn1;
-        const bool h0 = slab(A.x, A.y, A.z, A.w, B.x, B.y, r, minT, tn0);
-        const bool h1 = slab(B.z, B.w, C.x, C.y, C.z, C.w, r, minT, tn1);
-        if (STATS) nbox += 2;
-        bool one_first;
-        if (ANY) {
-          one_first = ((M.z >> 2) & 1) == 0;
-        } else {
-          const int ax = M.z & 3;
-          const float da = pick3(ax, r.d[0], r.d[1], r.d[2]);
-          one_first = !(da > 0.f);
-        }
-        const int fref = one_first ? M.y : M.x;
-        const int bref = one_first ? M.x : M.y;
-        const bool fh = one_first ? h1 : h0;
-        const bool bh = one_first ? h0 : h1;
-        const float btn = one_first ? tn0 : tn1;
-        if (fh) {
-          if (bh) {
-            if (sp - base == S) {
-              const int sl = base & (S - 1);
-              spill[(int64_t)base * spill_stride + gtid] =
-                  make_int2(s_ref[sl * kBlock + tid], __float_as_int(s_tn[sl * kBlock + tid]));
-              ++base;
-            }
-            const int sl = sp & (S - 1);
-            s_ref[sl * kBlock + tid] = bref;
-            s_tn[sl * kBlock + tid] = btn;
-            ++sp;
-          }
-          cur = fref;
-          continue;
-        }
-        if (bh) {
-          cur = bref;
-          continue;
-        }
-      } else {
-        const int code = ~cur;
-        const int first = code >> kLeafCountBits;
-        int cnt = code & kLeafCountInline;
-        if (cnt == kLeafCountInline) cnt = sc.big_count[first];
-        bool done = false;
-        for (int k = 0; k < cnt; ++k) {
-          if (STATS) ++ntri;
-          float t, u, v;
-          if (prim_test<PRIM>(sc.tris, first + k, r, minT, t, u, v)) {
-            minT = t;
-            best = first + k;
-            bu = u;
-            bv = v;
-            if (ANY) {
-              done = true;
-              break;
-            }
-          }
-        }
-        if (done) break;
-      }
-      go = false;
-      while (sp > 0) {
-        int ref;
-        float t;
-        if (sp == base) {
-          --sp;
-          --base;
-          const int2 e = spill[(int64_t)sp * spill_stride + gtid];
-          ref = e.x;
-          t = __int_as_float(e.y);
-        } else {
-          --sp;
-          const int sl = sp & (S - 1);
-          ref = s_ref[sl * kBlock + tid];
-          t = s_tn[sl * kBlock + tid];
-        }
-        if (t <= minT) {
-          cur = ref;
-          go = true;
-          break;
-        }
-      }
-    }
-    const bool hit = best >= 0;
-    store_hit(hits, ri, hit ? sc.obj_ids[best] : -1, (ANY && hit) ? kEps : minT, bu, bv, hit);
-  }
+  LaneStack<S, kBlock> st(s_ring, reinterpret_cast<unsigned long long*>(spill), spill_stride,
+                          gtid, tid);
+  const GRays gr = global_view(rays);
+  const GHits gh = global_view(hits);
+  for (int64_t ri = gtid; ri < n; ri += stride)
+    trace_store<ANY, PRIM, kLayoutPair, S, kBlock, STATS>(sc, root, gr, gh, ri, st, nbox, ntri);
   if (STATS && counters) {
     atomicAdd(counters + 0, nbox);
     atomicAdd(counters + 1, ntri);

@@ -15,6 +15,33 @@ namespace mh {
 constexpr float kEps = 1.e-4f;   // T_EPSILON
 constexpr float kMaxT = 1.e19f;  // MAXT
 
+// Explicit address spaces. Pointers read from memory (batch tables) are otherwise
+// generic, and the compiler then uses flat instructions; an LDS and a global access on
+// the two sides of a branch can also be merged into one flat access. Typed pointers keep
+// every access a global_* or ds_* instruction.
+#define MH_GLOBAL __attribute__((address_space(1)))
+#define MH_LDS __attribute__((address_space(3)))
+
+// Ray and hit arrays as global-memory views.
+struct GRays {
+  const MH_GLOBAL float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+  const MH_GLOBAL uint8_t* masked;
+};
+struct GHits {
+  MH_GLOBAL int32_t* prim;
+  MH_GLOBAL float *t, *u, *v;
+  MH_GLOBAL uint8_t* occluded;
+};
+static __device__ __forceinline__ GRays global_view(const DevRays& r) {
+  using F = const MH_GLOBAL float*;
+  return GRays{(F)r.ox, (F)r.oy, (F)r.oz, (F)r.dx, (F)r.dy, (F)r.dz, (F)r.tmax,
+               (const MH_GLOBAL uint8_t*)r.masked};
+}
+static __device__ __forceinline__ GHits global_view(const DevHits& h) {
+  using F = MH_GLOBAL float*;
+  return GHits{(MH_GLOBAL int32_t*)h.prim, (F)h.t, (F)h.u, (F)h.v, (MH_GLOBAL uint8_t*)h.occluded};
+}
+
 // a, b or c for k = 0, 1, 2, chosen with bit masks so that the compiler cannot turn the
 // choice into an indexed (scratch) access.
 static __device__ __forceinline__ float pick3(int k, float a, float b, float c) {
@@ -112,12 +139,16 @@ template <int PRIM>
 static __device__ __forceinline__ bool prim_test(const float* tris, int slot, const Ray& r, float minT,
                                           float& t, float& u, float& v) {
   const float4* p = reinterpret_cast<const float4*>(tris + 12 * (size_t)slot);
-  const float4 A = p[0], B = p[1], C = p[2];
+  float4 A = p[0], B = p[1], C = p[2];
+  // All 48 bytes are fetched before the first rejection test (one round trip) instead of
+  // the compiler deferring the edge loads past the distance test.
+  asm volatile("" : "+v"(A.x), "+v"(A.y), "+v"(A.z), "+v"(A.w), "+v"(B.x), "+v"(B.y), "+v"(B.z),
+               "+v"(B.w), "+v"(C.x), "+v"(C.y), "+v"(C.z), "+v"(C.w));
   if (PRIM == 0) return ks_test(A, B, C, r, minT, t, u, v);
   return wald_test(A, B, C, r, minT, t, u, v);
 }
 
-static __device__ __forceinline__ void load_ray(const DevRays& rays, int64_t i, Ray& r) {
+static __device__ __forceinline__ void load_ray(const GRays& rays, int64_t i, Ray& r) {
   r.o[0] = rays.ox[i];
   r.o[1] = rays.oy[i];
   r.o[2] = rays.oz[i];
@@ -130,13 +161,20 @@ static __device__ __forceinline__ void load_ray(const DevRays& rays, int64_t i, 
   r.inv[2] = 1.0f / r.d[2];
 }
 
-static __device__ __forceinline__ void store_hit(const DevHits& h, int64_t i, int32_t prim, float t,
+static __device__ __forceinline__ void store_hit(const GHits& h, int64_t i, int32_t prim, float t,
                                           float u, float v, bool occ) {
   if (h.prim) h.prim[i] = prim;
   if (h.t) h.t[i] = t;
   if (h.u) h.u[i] = u;
   if (h.v) h.v[i] = v;
   if (h.occluded) h.occluded[i] = occ ? 1 : 0;
+}
+static __device__ __forceinline__ void load_ray(const DevRays& rays, int64_t i, Ray& r) {
+  load_ray(global_view(rays), i, r);
+}
+static __device__ __forceinline__ void store_hit(const DevHits& h, int64_t i, int32_t prim, float t,
+                                          float u, float v, bool occ) {
+  store_hit(global_view(h), i, prim, t, u, v, occ);
 }
 
 

@@ -42,10 +42,19 @@ def _device(batches):
 
 
 @pytest.mark.parametrize("cfg", [
-    {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 8, "refill": 16},
-    {"layout": 1, "stack_lds": 8, "block": 64, "blocks_per_cu": 4, "refill": 1},
-    {"layout": 0, "stack_lds": 32, "block": 128, "blocks_per_cu": 2, "refill": 64},
-    {"layout": 1, "stack_lds": 16, "block": 256, "blocks_per_cu": 1, "refill": 32},
+    # per-lane refill, pools of 64 and 256 rays
+    {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 8, "refill": 16, "chunk": 64},
+    {"layout": 1, "stack_lds": 8, "block": 64, "blocks_per_cu": 4, "refill": 1, "chunk": 256},
+    # whole-wave chunks from the counter, with and without the XCD remap
+    {"layout": 0, "stack_lds": 32, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 64},
+    {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 3, "refill": 64, "chunk": 1024,
+     "xcd_remap": 1},
+    {"layout": 1, "stack_lds": 16, "block": 256, "blocks_per_cu": 1, "refill": 32, "chunk": 128,
+     "xcd_remap": 1},
+    # static stride, no counter
+    {"layout": 0, "stack_lds": 16, "block": 64, "blocks_per_cu": 8, "refill": 64, "chunk": 0,
+     "xcd_remap": 1},
+    {"layout": 1, "stack_lds": 8, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 0},
 ])
 def test_frame_matches_oracle(oracle, small, cfg):
     v, i, batches = small
@@ -55,7 +64,10 @@ def test_frame_matches_oracle(oracle, small, cfg):
         dev = _device(batches)
         f = api.Frame(s, dev)
         f.set_config(cfg)
+        full = {"chunk": 64, "xcd_remap": 0, **cfg}
+        assert f.config() == full
         f.launch()
+        f.launch()  # a second launch checks that the counters reset themselves
         api.sync()
         for (k, rays), (dr, dh, any_hit) in zip(batches, dev):
             g = dh.download()
