@@ -205,7 +205,8 @@ typedef struct mh_frame mh_frame;
 
 int mh_frame_create(const mh_scene* scene, const mh_batch* batches, int n_batches,
                     mh_frame** out);
-/* Rewrites the batch table (n_batches must equal the frame's), on `stream`. */
+/* Rewrites the batch table (n_batches must equal the frame's), on `stream`. A change
+ * between all closest-hit, all any-hit and mixed batches drops the captured graph. */
 int mh_frame_set_batches(mh_frame* frame, const mh_batch* batches, int n_batches, void* stream);
 int mh_frame_set_config(mh_frame* frame, const mh_launch_config* cfg); /* NULL = default */
 int mh_frame_get_config(const mh_frame* frame, mh_launch_config* cfg);
@@ -263,19 +264,24 @@ int mh_gen_bounce_rays(const mh_scene* scene, const mh_rays* primary, const mh_h
 
 /* ---- Serving loop -----------------------------------------------------------------
  * No reference counterpart. A server keeps scenes resident and renders requests (a
- * camera over one of its scenes: closest-hit primary rays, optionally one shadow ray
- * per hit) with continuous batching: each round of its scheduler thread admits waiting
- * requests while their device buffers fit the HBM budget, then launches one persistent
- * batch trace per scene over a chunk of every admitted request's remaining rays (scenes
- * on separate streams, overlapping), and completes requests whose rays are done by
- * copying their hits to the host. Per-ray results equal mh_trace with packet_size 1. */
+ * camera over one of its scenes: closest-hit primary rays and, optionally, one shadow ray
+ * and one diffuse bounce ray per primary ray) with continuous batching: each round of its
+ * scheduler thread admits waiting requests in arrival order while their device buffers
+ * fit the HBM budget, then launches one persistent batch trace per scene over a chunk of
+ * every admitted request's remaining rays (scenes on separate streams, overlapping);
+ * requests whose rays are all traced have their results copied to pinned host staging
+ * and their device buffers freed. Per-ray results equal mh_gen_camera_rays +
+ * mh_trace (packet_size 1) + mh_gen_shadow_rays / mh_gen_bounce_rays + mh_trace. */
 typedef struct mh_server mh_server;
 
 typedef struct mh_server_opts {
-  size_t hbm_budget;   /* bytes of device request buffers admitted at once [1 GiB] */
-  int64_t chunk_rays;  /* rays of one request traced per round [1<<20] */
-  int max_active;      /* requests in flight [64] */
-  int shadows;         /* 1 = one shadow ray per primary hit toward `light` [0] */
+  size_t hbm_budget;   /* device bytes of request buffers in flight [1 GiB] */
+  size_t host_budget;  /* pinned host bytes of finished, uncollected results [hbm_budget/4] */
+  int64_t chunk_rays;  /* rays of one ray set of a request traced per round [1<<20] */
+  int max_active;      /* requests in flight [64, at most 128] */
+  int shadows;         /* 1 = one shadow ray per primary ray toward `light` (any-hit) [0] */
+  int bounces;         /* 1 = one diffuse bounce ray per primary ray (closest-hit), as
+                          mh_gen_bounce_rays with seed = camera.seed + 1 [0] */
   float light[3];
   mh_launch_config launch; /* zero = default configuration */
 } mh_server_opts;
@@ -285,33 +291,46 @@ typedef struct mh_request {
   mh_camera camera;
 } mh_request;
 
+/* Result arrays of one request, xres*yres*spp entries each in camera ray order; any
+ * pointer may be NULL. */
+typedef struct mh_request_result {
+  int32_t* prim;        /* primary closest hit: triangle id, MH_NO_HIT */
+  float* t;
+  uint8_t* occluded;    /* shadow ray of each primary ray (shadows) */
+  int32_t* bounce_prim; /* bounce ray of each primary ray (bounces) */
+  float* bounce_t;
+} mh_request_result;
+
 typedef struct mh_request_info {
   uint64_t id;
   int status;        /* MH_OK or the error that ended the request */
-  int64_t rays;      /* rays traced (primary + shadow) */
+  int64_t rays;      /* rays traced (primary + shadow + bounce) */
   double submit_ms;  /* server clock */
   double admit_ms;
-  double done_ms;    /* hits on the host */
+  double done_ms;    /* results in host memory */
 } mh_request_info;
 
 typedef struct mh_server_stats {
   uint64_t submitted, completed, rejected;
   int64_t rays;        /* rays traced since creation */
-  double busy_ms;      /* wall time with at least one request in flight */
+  double busy_ms;      /* wall time of scheduler rounds */
   int active, pending;
   size_t hbm_in_use;
+  size_t host_in_use;
 } mh_server_stats;
 
 int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_server_opts* opts,
                      mh_server** out);
-/* Queues a request; returns at once. Requests larger than the budget are rejected. */
+/* Queues a request; returns at once. Requests larger than the budgets are rejected. The
+ * scenes must not be changed while a request over them is in flight. */
 int mh_server_submit(mh_server* server, const mh_request* req, uint64_t* id);
-/* Waits for a request; copies its hits (any pointer may be NULL): prim and t per primary
- * ray (xres*yres*spp, the camera ray order), occluded per primary ray (shadows). */
-int mh_server_wait(mh_server* server, uint64_t id, int32_t* prim, float* t, uint8_t* occluded,
+/* Waits for a request and copies its results out (out may be NULL); each id once. */
+int mh_server_wait(mh_server* server, uint64_t id, const mh_request_result* out,
                    mh_request_info* info);
 int mh_server_get_stats(mh_server* server, mh_server_stats* stats);
 double mh_server_clock_ms(const mh_server* server);
+/* Finishes the requests already submitted, then stops; results not yet collected are
+ * discarded. */
 int mh_server_destroy(mh_server* server);
 
 /* Device memory helpers for callers without their own allocator. */

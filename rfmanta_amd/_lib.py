@@ -87,12 +87,18 @@ class BuildOpts(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("chunk_rays", C.c_int64), ("max_active", C.c_int),
-                ("shadows", C.c_int), ("light", C.c_float * 3), ("launch", LaunchConfig)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("host_budget", C.c_size_t),
+                ("chunk_rays", C.c_int64), ("max_active", C.c_int), ("shadows", C.c_int),
+                ("bounces", C.c_int), ("light", C.c_float * 3), ("launch", LaunchConfig)]
 
 
 class Request(C.Structure):
     _fields_ = [("scene", C.c_int), ("camera", Camera)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("prim", C.c_void_p), ("t", C.c_void_p), ("occluded", C.c_void_p),
+                ("bounce_prim", C.c_void_p), ("bounce_t", C.c_void_p)]
 
 
 class RequestInfo(C.Structure):
@@ -103,7 +109,7 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("rays", C.c_int64), ("busy_ms", C.c_double), ("active", C.c_int),
-                ("pending", C.c_int), ("hbm_in_use", C.c_size_t)]
+                ("pending", C.c_int), ("hbm_in_use", C.c_size_t), ("host_in_use", C.c_size_t)]
 
 
 # Every symbol include/mh.h declares, with its ctypes signature.
@@ -152,7 +158,7 @@ _SIGS = {
     "mh_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(ServerOpts),
                                    C.POINTER(C.c_void_p)]),
     "mh_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
-    "mh_server_wait": (C.c_int, [_P, C.c_uint64, _P, _P, _P, C.POINTER(RequestInfo)]),
+    "mh_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(RequestResult), C.POINTER(RequestInfo)]),
     "mh_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "mh_server_clock_ms": (C.c_double, [_P]),
     "mh_server_destroy": (C.c_int, [_P]),

@@ -365,16 +365,17 @@ def gen_bounce_rays(scene: "Scene", primary: DeviceRays, hits: DeviceHits, seed:
 class Server:
     """Serving loop over resident scenes (mh_server_*): submit camera requests, wait for hits."""
 
-    def __init__(self, scenes: list, hbm_budget: int = 0, chunk_rays: int = 0,
-                 max_active: int = 0, shadows: bool = False, light=(0.0, 0.0, 0.0),
-                 launch: dict | None = None):
+    def __init__(self, scenes: list, hbm_budget: int = 0, host_budget: int = 0,
+                 chunk_rays: int = 0, max_active: int = 0, shadows: bool = False,
+                 bounces: bool = False, light=(0.0, 0.0, 0.0), launch: dict | None = None):
         self.scenes = list(scenes)
         o = L.ServerOpts()
-        o.hbm_budget, o.chunk_rays, o.max_active, o.shadows = hbm_budget, chunk_rays, max_active, int(shadows)
+        o.hbm_budget, o.host_budget, o.chunk_rays = hbm_budget, host_budget, chunk_rays
+        o.max_active, o.shadows, o.bounces = max_active, int(shadows), int(bounces)
         o.light[:] = [float(x) for x in light]
         if launch is not None:
             o.launch = L.LaunchConfig.from_dict(launch)
-        self.shadows = bool(shadows)
+        self.shadows, self.bounces = bool(shadows), bool(bounces)
         arr = (C.c_void_p * len(self.scenes))(*[sc.h for sc in self.scenes])
         h = C.c_void_p()
         L.check(L.load().mh_server_create(arr, len(self.scenes), C.byref(o), C.byref(h)))
@@ -388,19 +389,28 @@ class Server:
         self._sizes[rid.value] = cam.xres * cam.yres * cam.spp
         return rid.value
 
-    def wait(self, rid: int) -> dict:
-        n = self._sizes.pop(rid, 0)
-        prim = np.empty(n, np.int32)
-        t = np.empty(n, np.float32)
-        occ = np.empty(n, np.uint8) if self.shadows else None
+    def wait(self, rid: int, results: bool = True) -> dict:
+        """Waits for request `rid`; returns its info and (results=True) its result arrays."""
+        n = self._sizes.pop(rid, 0) if results else 0
+        out = {}
+        res = L.RequestResult()
+        if results:
+            out["prim"], out["t"] = np.empty(n, np.int32), np.empty(n, np.float32)
+            res.prim, res.t = out["prim"].ctypes.data, out["t"].ctypes.data
+            if self.shadows:
+                out["occluded"] = np.empty(n, np.uint8)
+                res.occluded = out["occluded"].ctypes.data
+            if self.bounces:
+                out["bounce_prim"] = np.empty(n, np.int32)
+                out["bounce_t"] = np.empty(n, np.float32)
+                res.bounce_prim = out["bounce_prim"].ctypes.data
+                res.bounce_t = out["bounce_t"].ctypes.data
+        else:
+            self._sizes.pop(rid, None)
         info = L.RequestInfo()
-        L.check(L.load().mh_server_wait(self.h, rid, prim.ctypes.data, t.ctypes.data,
-                                        occ.ctypes.data if occ is not None else None,
-                                        C.byref(info)))
-        out = {"prim": prim, "t": t, "id": info.id, "status": info.status, "rays": info.rays,
-               "submit_ms": info.submit_ms, "admit_ms": info.admit_ms, "done_ms": info.done_ms}
-        if occ is not None:
-            out["occluded"] = occ
+        L.check(L.load().mh_server_wait(self.h, rid, C.byref(res), C.byref(info)))
+        out.update(id=info.id, status=info.status, rays=info.rays, submit_ms=info.submit_ms,
+                   admit_ms=info.admit_ms, done_ms=info.done_ms)
         return out
 
     def stats(self) -> dict:

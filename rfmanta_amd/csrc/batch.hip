@@ -1,358 +1,20 @@
-// Persistent multi-batch traversal for gfx950 (wave64).
-//
-// k_trace_batch consumes a table of ray batches (each its own SoA arrays, size and
-// closest-hit/any-hit flag) in one launch. Rays are numbered across batches by the
-// table's prefix offsets. Three schedules (persistent workgroups in all of them):
-//   whole-wave chunks (refill 64, chunk C > 0): a wave traces C consecutive rays, 64 at a
-//     time, so the rays of a wave stay as coherent as their batch. Its first chunk is
-//     assigned statically (wave id), later ones come from a shared counter, one atomic
-//     per C rays, which balances the load between waves;
-//   static stride (refill 64, chunk 0): the same without the counter;
-//   per-lane refill (refill < 64): each wave keeps a pool of ray numbers taken C at a
-//     time from the counter; a lane whose ray has finished takes the next number as soon
-//     as at least `refill` lanes of its wave are idle, so lanes do not wait for the
-//     wave's slowest ray (ballot + popcount hand-out, no LDS traffic).
-// The last wave to leave resets the counters, so consecutive launches (and graph
-// replays) need no clearing pass.
-//
-// Two node layouts, both giving the reference's one-ray results and visit order
-// (DynBVH::intersectNode on a one-ray packet, Model/Groups/DynBVH.cc:274-404):
-//   PAIR: 64-byte PairNodes hold both child boxes; the far child is pushed with its entry
-//         distance and skipped at pop time if that exceeds the current minT.
-//   REF:  the reference's 32-byte records; each node's own box is tested when it is
-//         visited, exactly as firstIntersects does.
-// Triangle tests and arithmetic are those of trace.hip (shared header).
-
-#include <hip/hip_runtime.h>
-
-#include <cfloat>
-
-#include "mh_device.h"
-#include "trace_common.h"
-#include "traverse.h"
+// Entry of the persistent multi-batch trace (batch_kernel.h): argument checks and the
+// choice of the launcher for the scene's triangle test and the node layout.
+#include "batch_kernel.h"
 
 namespace mh {
-namespace {
-
-constexpr int kMaxBatches = 256;  // MH_MAX_BATCHES
-constexpr int kModeLane = 0;        // per-lane refill from a wave pool
-constexpr int kModeWave = 1;        // whole-wave chunks from the shared counter
-constexpr int kModeWaveStatic = 2;  // whole-wave chunks on a static stride, no counter
-
-// Last batch whose first ray number is <= g (empty batches are skipped over).
-__device__ __forceinline__ int find_batch(const int64_t* s_start, int nb, int64_t g) {
-  int lo = 0, hi = nb - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s_start[mid] <= g)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-// Rays [pos, gend) of one batch whose first ray number is bstart, 64 per step of the wave.
-template <bool ANY, int PRIM, int LAYOUT, int S, int BLOCK>
-__device__ __forceinline__ void trace_span(const DevScene& sc, const RefNode& root,
-                                           const GRays& rays, const GHits& hits,
-                                           int64_t bstart, int64_t pos, int64_t gend, int lane,
-                                           LaneStack<S, BLOCK>& st) {
-  unsigned long long nbox = 0, ntri = 0;
-  for (int64_t b = pos; b < gend; b += 64) {
-    const int64_t g = b + lane;
-    if (g < gend)
-      trace_store<ANY, PRIM, LAYOUT, S, BLOCK, false>(sc, root, rays, hits, g - bstart, st, nbox,
-                                                      ntri);
-  }
-}
-
-// Per-lane state of the lane-refill schedule: one traversal advanced a node at a time.
-struct Lane {
-  Ray r;
-  RayHit h;
-  int cur;
-  int bi;      // batch index
-  int64_t li;  // ray index inside the batch
-  bool any;
-};
-
-// One traversal step of traverse.h's loops with a run-time closest/any-hit flag;
-// returns false when the ray is finished.
-template <int PRIM, int LAYOUT, int S, int BLOCK>
-__device__ __forceinline__ bool step(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
-  unsigned long long ntri = 0;
-  if (LAYOUT == kLayoutPair) {
-    if (L.cur >= 0) {
-      const float4* np = reinterpret_cast<const float4*>(sc.pairs + L.cur);
-      const float4 A = np[0], B = np[1], C = np[2];
-      const int4 M = reinterpret_cast<const int4*>(np)[3];
-      float tn0, tn1;
-      const bool h0 = slab(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
-      const bool h1 = slab(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
-      const bool one_first =
-          L.any ? ((M.z >> 2) & 1) == 0 : !(pick3(M.z & 3, L.r.d[0], L.r.d[1], L.r.d[2]) > 0.f);
-      const int fref = one_first ? M.y : M.x;
-      const int bref = one_first ? M.x : M.y;
-      const bool fh = one_first ? h1 : h0;
-      const bool bh = one_first ? h0 : h1;
-      if (fh) {
-        if (bh) st.push(bref, one_first ? tn0 : tn1);
-        L.cur = fref;
-        return true;
-      }
-      if (bh) {
-        L.cur = bref;
-        return true;
-      }
-    } else {
-      const int code = ~L.cur;
-      const int first = code >> kLeafCountBits;
-      int cnt = code & kLeafCountInline;
-      if (cnt == kLeafCountInline) cnt = sc.big_count[first];
-      const bool done = L.any ? leaf<true, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)
-                              : leaf<false, PRIM, false>(sc, first, cnt, L.r, L.h, ntri);
-      if (done) return false;
-    }
-    while (!st.empty()) {
-      int ref;
-      float tn;
-      st.pop(ref, tn);
-      if (tn <= L.h.minT) {
-        L.cur = ref;
-        return true;
-      }
-    }
-    return false;
-  } else {
-    const RefNode nd = sc.refs[L.cur];
-    float tn;
-    if (slab(nd.box.lo[0], nd.box.hi[0], nd.box.lo[1], nd.box.hi[1], nd.box.lo[2], nd.box.hi[2],
-             L.r, L.h.minT, tn)) {
-      if (nd.children != 0) {
-        const bool done = L.any ? leaf<true, PRIM, false>(sc, nd.child, nd.children, L.r, L.h, ntri)
-                                : leaf<false, PRIM, false>(sc, nd.child, nd.children, L.r, L.h, ntri);
-        if (done) return false;
-      } else {
-        const int front = L.any ? ((nd.bits & 1) ? 0 : 1)
-                                : (pick3(nd.axis, L.r.d[0], L.r.d[1], L.r.d[2]) > 0.f ? 0 : 1);
-        st.push(nd.child + 1 - front, 0.f);
-        L.cur = nd.child + front;
-        return true;
-      }
-    }
-    if (st.empty()) return false;
-    st.pop(L.cur, tn);
-    return true;
-  }
-}
-
-// Starts ray g (global number); returns false if it finished immediately.
-template <int PRIM, int LAYOUT, int S, int BLOCK>
-__device__ __forceinline__ bool start_ray(const DevScene& sc, const RefNode& root,
-                                          const BatchDev* batches, const int64_t* s_start, int nb,
-                                          int64_t g, Lane& L, LaneStack<S, BLOCK>& st) {
-  L.bi = find_batch(s_start, nb, g);
-  L.li = g - s_start[L.bi];
-  const BatchDev& B = batches[L.bi];
-  L.any = (B.flags & 1u) != 0;
-  if (B.rays.masked && B.rays.masked[L.li]) {
-    store_hit(B.hits, L.li, -2, -kMaxT, 0.f, 0.f, true);
-    return false;
-  }
-  load_ray(B.rays, L.li, L.r);
-  L.h = RayHit{B.rays.tmax[L.li], -1, 0.f, 0.f};
-  st.reset();
-  if (LAYOUT == kLayoutPair) {
-    float tn;
-    if (!slab(root.box.lo[0], root.box.hi[0], root.box.lo[1], root.box.hi[1], root.box.lo[2],
-              root.box.hi[2], L.r, L.h.minT, tn)) {
-      store_hit(B.hits, L.li, -1, L.h.minT, 0.f, 0.f, false);
-      return false;
-    }
-    L.cur = sc.root_ref;
-  } else {
-    L.cur = 0;
-  }
-  return true;
-}
-
-__device__ __forceinline__ void finish(const BatchDev* batches, const DevScene& sc, const Lane& L) {
-  const BatchDev& B = batches[L.bi];
-  const bool hit = L.h.best >= 0;
-  store_hit(B.hits, L.li, hit ? sc.obj_ids[L.h.best] : -1, (L.any && hit) ? kEps : L.h.minT,
-            L.h.u, L.h.v, hit);
-}
-
-template <int PRIM, int LAYOUT, int S, int BLOCK, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_trace_batch(DevScene sc,
-                                                       const BatchTable* __restrict__ table,
-                                                       unsigned long long* ctr, int2* spill,
-                                                       int64_t spill_stride, int nb_launch,
-                                                       int refill, int chunk, int xcd_remap) {
-  __shared__ unsigned long long s_ring[S * BLOCK];
-  extern __shared__ int64_t s_start[];  // nb entries (dynamic LDS)
-  // The dynamic LDS holds nb_launch offsets; a table with more entries is cut to that.
-  const int nb = table->nb < nb_launch ? table->nb : nb_launch;
-  const int64_t total = nb == table->nb ? table->total : table->batches[nb - 1].start +
-                                                             table->batches[nb - 1].count;
-  const BatchDev* batches = table->batches;
-  for (int i = threadIdx.x; i < nb; i += BLOCK) s_start[i] = batches[i].start;
-  __syncthreads();
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int64_t gtid = (int64_t)blockIdx.x * BLOCK + tid;  // spill column (physical)
-  // Workgroups are dealt to the 8 XCDs round-robin; with xcd_remap the logical order
-  // gives XCD x the x-th eighth of the workgroups, so its static shares are contiguous.
-  int blk = blockIdx.x;
-  if (xcd_remap && (gridDim.x & 7) == 0) blk = (blk & 7) * (gridDim.x >> 3) + (blk >> 3);
-  const int64_t n_waves = (int64_t)gridDim.x * (BLOCK / 64);
-  const int64_t wave_id = ((int64_t)blk * BLOCK + tid) >> 6;
-  const RefNode root = sc.refs[0];
-  LaneStack<S, BLOCK> st(s_ring, reinterpret_cast<unsigned long long*>(spill), spill_stride,
-                         gtid, tid);
-
-  if (MODE != kModeLane) {
-    // A wave traces C consecutive ray numbers, 64 at a time, so its rays stay as coherent
-    // as their batch. Its first chunk is wave_id * C; later chunks come from the counter
-    // (dynamic) or the next stride (static). Each span of one batch inside a chunk runs
-    // with that batch's arrays and its closest/any-hit kind fixed for the whole wave.
-    const int64_t C = MODE == kModeWaveStatic ? 64 : (int64_t)chunk;
-    const int64_t static_part = n_waves * C;
-    int64_t next = wave_id * C;
-    while (next < total) {
-      const int64_t stop = next + C < total ? next + C : total;
-      int64_t pos = next;
-      while (pos < stop) {
-        const int bi = __builtin_amdgcn_readfirstlane(find_batch(s_start, nb, pos));
-        const BatchDev& B = batches[bi];  // uniform: scalar loads
-        const GRays rays = global_view(B.rays);
-        const GHits hits = global_view(B.hits);
-        const int64_t bstart = s_start[bi];
-        const int64_t bend = bstart + B.count;
-        const int64_t gend = stop < bend ? stop : bend;
-        if (B.flags & 1u)
-          trace_span<true, PRIM, LAYOUT, S, BLOCK>(sc, root, rays, hits, bstart, pos, gend, lane, st);
-        else
-          trace_span<false, PRIM, LAYOUT, S, BLOCK>(sc, root, rays, hits, bstart, pos, gend, lane, st);
-        pos = gend;
-      }
-      if (MODE == kModeWaveStatic) {
-        next += static_part;
-      } else {
-        unsigned long long x = 0;
-        if (lane == 0) x = atomicAdd(ctr, (unsigned long long)C);
-        next = (int64_t)__shfl((long long)x, 0, 64) + static_part;
-      }
-    }
-  } else {
-    const int64_t C = chunk;
-    Lane L;
-    bool active = false;
-    bool exhausted = false;
-    int64_t pool_next = 0, pool_end = 0;  // wave-uniform
-
-    for (;;) {
-      const unsigned long long idle = __ballot(!active);
-      const int n_idle = __popcll(idle);
-      if (n_idle == 64 && exhausted) break;
-      if (n_idle >= refill && !exhausted) {
-        const int rank = __popcll(idle & ((1ull << lane) - 1ull));
-        const int64_t avail = pool_end - pool_next;
-        int64_t g = -1;
-        if (!active && rank < avail) g = pool_next + rank;
-        if (n_idle > avail) {
-          unsigned long long b = 0;
-          if (lane == 0) b = atomicAdd(ctr, (unsigned long long)C);
-          const int64_t base = (int64_t)__shfl((long long)b, 0, 64);
-          if (base >= total) {
-            exhausted = true;
-            pool_next = pool_end = total;
-          } else {
-            const int64_t end = base + C < total ? base + C : total;
-            if (!active && rank >= avail && base + (rank - avail) < end) g = base + (rank - avail);
-            const int64_t taken = n_idle - avail;
-            pool_next = base + taken < end ? base + taken : end;
-            pool_end = end;
-          }
-        } else {
-          pool_next += n_idle;
-        }
-        if (g >= 0) active = start_ray<PRIM, LAYOUT, S, BLOCK>(sc, root, batches, s_start, nb, g, L, st);
-      }
-      if (active) {
-        if (!step<PRIM, LAYOUT, S, BLOCK>(sc, L, st)) {
-          finish(batches, sc, L);
-          active = false;
-        }
-      }
-    }
-  }
-  if (MODE != kModeWaveStatic && lane == 0) {
-    // The last wave out clears the counters for the next launch.
-    const unsigned long long old = atomicAdd(ctr + 1, 1ull);
-    if (old == (unsigned long long)n_waves - 1) {
-      atomicExch(ctr + 0, 0ull);
-      atomicExch(ctr + 1, 0ull);
-    }
-  }
-}
-
-template <int PRIM, int LAYOUT, int BLOCK, int MODE>
-int launch_w(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
-  dim3 g(bl.blocks), b(BLOCK);
-  const size_t lds = sizeof(int64_t) * (size_t)bl.nb;  // batch start offsets
-  int2* sp = reinterpret_cast<int2*>(bl.spill);
-  switch (bl.stack_lds) {
-    case 8:
-      hipLaunchKernelGGL((k_trace_batch<PRIM, LAYOUT, 8, BLOCK, MODE>), g, b, lds, s, sc, bl.table,
-                         bl.counters, sp, bl.spill_stride, bl.nb, bl.refill, bl.chunk, bl.xcd_remap);
-      break;
-    case 32:
-      hipLaunchKernelGGL((k_trace_batch<PRIM, LAYOUT, 32, BLOCK, MODE>), g, b, lds, s, sc, bl.table,
-                         bl.counters, sp, bl.spill_stride, bl.nb, bl.refill, bl.chunk, bl.xcd_remap);
-      break;
-    default:
-      hipLaunchKernelGGL((k_trace_batch<PRIM, LAYOUT, 16, BLOCK, MODE>), g, b, lds, s, sc, bl.table,
-                         bl.counters, sp, bl.spill_stride, bl.nb, bl.refill, bl.chunk, bl.xcd_remap);
-      break;
-  }
-  return (int)hipGetLastError();
-}
-
-// refill 64: whole-wave chunks (chunk 0: static stride); otherwise per-lane refill.
-template <int PRIM, int LAYOUT, int BLOCK>
-int launch_s(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
-  if (bl.refill < 64) return launch_w<PRIM, LAYOUT, BLOCK, kModeLane>(sc, bl, s);
-  return bl.chunk == 0 ? launch_w<PRIM, LAYOUT, BLOCK, kModeWaveStatic>(sc, bl, s)
-                       : launch_w<PRIM, LAYOUT, BLOCK, kModeWave>(sc, bl, s);
-}
-
-template <int PRIM, int LAYOUT>
-int launch_b(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
-  switch (bl.block) {
-    case 64:
-      return launch_s<PRIM, LAYOUT, 64>(sc, bl, s);
-    case 128:
-      return launch_s<PRIM, LAYOUT, 128>(sc, bl, s);
-    default:
-      return launch_s<PRIM, LAYOUT, 256>(sc, bl, s);
-  }
-}
-
-}  // namespace
 
 int launch_trace_batch(const DevScene& sc, const BatchLaunch& bl, void* stream) {
   if (bl.nb <= 0 || bl.nb > kMaxBatches) return (int)hipErrorInvalidValue;
   if (bl.chunk < 0 || bl.chunk % 64 != 0 || (bl.refill < 64 && bl.chunk == 0))
     return (int)hipErrorInvalidValue;
+  if (bl.kinds < kKindMixed || bl.kinds > kKindAny) return (int)hipErrorInvalidValue;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (sc.prim_kind == 1)
-    return bl.layout == kLayoutRef ? launch_b<1, kLayoutRef>(sc, bl, s)
-                                   : launch_b<1, kLayoutPair>(sc, bl, s);
-  return bl.layout == kLayoutRef ? launch_b<0, kLayoutRef>(sc, bl, s)
-                                 : launch_b<0, kLayoutPair>(sc, bl, s);
+    return bl.layout == kLayoutRef ? launch_batch_wald_ref(sc, bl, s)
+                                   : launch_batch_wald_pair(sc, bl, s);
+  return bl.layout == kLayoutRef ? launch_batch_ks_ref(sc, bl, s)
+                                 : launch_batch_ks_pair(sc, bl, s);
 }
 
 }  // namespace mh

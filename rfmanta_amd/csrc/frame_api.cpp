@@ -71,6 +71,9 @@ namespace {
 const mh_launch_config kDefault = {MH_LAYOUT_PAIR, 8, 256, 16, 64, 256, 1};
 
 
+// Kernel specialisation for a table: closest-hit only, any-hit only, or mixed.
+int kinds_of(int nb, int n_any) { return n_any == 0 ? 1 : (n_any == nb ? 2 : 0); }
+
 int build_table(const mh_batch* b, int n, std::vector<uint8_t>& out, int64_t& total, int& n_any) {
   if (!b || n < 1 || n > MH_MAX_BATCHES)
     return fail(MH_E_INVALID, "batch count %d not in 1..%d", n, MH_MAX_BATCHES);
@@ -142,6 +145,7 @@ int launch(mh_frame* f, void* stream) {
   bl.refill = f->cfg.refill;
   bl.chunk = f->cfg.chunk;
   bl.xcd_remap = f->cfg.xcd_remap;
+  bl.kinds = kinds_of(f->nb, f->n_any);
   HIP_OK((hipError_t)mh::launch_trace_batch(f->scene->dev(), bl, stream));
   return MH_OK;
 }
@@ -269,6 +273,8 @@ int mh_frame_set_batches(mh_frame* f, const mh_batch* batches, int n_batches, vo
   HIP_OK(hipMemcpyAsync(f->table.p, host.data(), host.size(), hipMemcpyHostToDevice,
                         reinterpret_cast<hipStream_t>(stream)));
   HIP_OK(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+  // A captured graph holds the kernel specialised for the old mix of batch kinds.
+  if (kinds_of(f->nb, n_any) != kinds_of(f->nb, f->n_any)) f->drop_graph();
   f->total = total;
   f->n_any = n_any;
   return MH_OK;
@@ -358,13 +364,8 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
   }
   mh_launch_config best = f->cfg;
   float best_ms = 1e30f;
-  const int layouts[2] = {MH_LAYOUT_PAIR, MH_LAYOUT_REF};
-  const int stacks[3] = {8, 16, 32};
-  const int blocks[3] = {64, 128, 256};
-  const int bpcs[6] = {2, 4, 8, 12, 16, 24};
-  // Stage 1: layout x stack x block x workgroups per CU, whole-wave chunks of 256 rays.
   auto trial = [&](const mh_launch_config& c) -> int {
-    if (mh_internal::check_launch_config(c) != MH_OK) return MH_OK;  // outside the LDS budget: skip
+    if (mh_internal::check_launch_config(c) != MH_OK) return MH_OK;  // outside the limits
     int r = prepare(f, c);
     if (r) return r;
     if ((r = launch(f, nullptr))) return r;  // warm-up
@@ -376,25 +377,34 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
     }
     return MH_OK;
   };
-  for (int l : layouts)
-    for (int s : stacks)
-      for (int b : blocks)
-        for (int p : bpcs)
-          if ((rc = trial(mh_launch_config{l, s, b, p, 64, 256, 1}))) return rc;
-  // Stage 2: schedule at the best shape: chunk size (0 = static stride) x XCD remap.
-  const int chunks[6] = {0, 64, 128, 256, 512, 1024};
+  // Stage 1: launch shape (stack ring x workgroup size x workgroups per CU) jointly with
+  // the schedule (static stride, small and large counter chunks), PAIR layout.
+  for (int s : {8, 16})
+    for (int b : {64, 128, 256})
+      for (int p : {4, 8, 12, 16, 24})
+        for (int c : {0, 64, 256})
+          if ((rc = trial(mh_launch_config{MH_LAYOUT_PAIR, s, b, p, 64, c, 0}))) return rc;
+  // Stage 2: chunk size x XCD remap at the best shape.
   mh_launch_config base = best;
-  for (int c : chunks)
+  for (int c : {0, 64, 128, 256, 512, 1024})
     for (int x = 0; x < 2; ++x) {
       mh_launch_config t = base;
       t.chunk = c;
       t.xcd_remap = x;
       if ((rc = trial(t))) return rc;
     }
-  // Stage 3: per-lane refill thresholds.
-  const int refills[5] = {4, 8, 16, 32, 48};
+  // Stage 3: the reference's 32-byte node layout and the deepest LDS ring.
   base = best;
-  for (int r : refills)
+  for (int l : {MH_LAYOUT_PAIR, MH_LAYOUT_REF})
+    for (int s : {8, 16, 32}) {
+      mh_launch_config t = base;
+      t.layout = l;
+      t.stack_lds = s;
+      if ((rc = trial(t))) return rc;
+    }
+  // Stage 4: per-lane refill thresholds.
+  base = best;
+  for (int r : {8, 16, 32})
     for (int c : {64, 256}) {
       mh_launch_config t = base;
       t.refill = r;

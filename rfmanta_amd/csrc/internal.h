@@ -81,7 +81,8 @@ struct mh_scene {
   int root_ref = 0;
   uint32_t n_inner = 0, n_leaves = 0, depth = 0;
 
-  DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, parent, slot, leaves, arrive;
+  DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, slot, leaves, levels;
+  std::vector<int64_t> level_off;  // inner records of depth d: levels[level_off[d], level_off[d+1])
 
   std::mutex spill_mu;  // guards the per-stream spill areas
   std::mutex stage_mu;  // serialises mh_trace_host calls on the staging buffer
@@ -92,7 +93,7 @@ struct mh_scene {
   }
   size_t device_bytes() const {
     return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
-           d_base.bytes + d_idx.bytes + parent.bytes + slot.bytes + leaves.bytes + arrive.bytes;
+           d_base.bytes + d_idx.bytes + slot.bytes + leaves.bytes + levels.bytes;
   }
   mh::DevScene dev() const {
     mh::DevScene d;

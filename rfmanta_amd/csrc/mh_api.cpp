@@ -87,8 +87,9 @@ int upload(mh_scene* s) {
   const std::vector<mh::RefNode>& nodes = s->bvh.nodes;
   const uint32_t nrec = (uint32_t)nodes.size();
   const uint32_t T = s->nt;
-  std::vector<int32_t> pair_of(nrec, -1), parent(nrec, -1), slot(nrec, -1);
+  std::vector<int32_t> pair_of(nrec, -1), slot(nrec, -1);
   std::vector<int32_t> leaves;
+  std::vector<std::vector<int32_t>> by_depth;  // inner records per depth, preorder
   std::vector<int32_t> big(T, 0);
   std::vector<int32_t> order;  // inner records, preorder
   struct It {
@@ -117,8 +118,8 @@ int upload(mh_scene* s) {
     pair_of[it.rec] = (int32_t)order.size();
     order.push_back(it.rec);
     if (order.size() > nrec) return fail(MH_E_INVALID, "node graph is not a tree");
-    parent[n.child] = it.rec;
-    parent[n.child + 1] = it.rec;
+    if (by_depth.size() <= it.depth) by_depth.resize(it.depth + 1);
+    by_depth[it.depth].push_back(it.rec);
     st.push_back({n.child + 1, it.depth + 1});
     st.push_back({n.child, it.depth + 1});
   }
@@ -151,6 +152,12 @@ int upload(mh_scene* s) {
   s->n_inner = (uint32_t)order.size();
   s->n_leaves = (uint32_t)leaves.size();
   s->depth = max_depth;
+  std::vector<int32_t> levels;
+  s->level_off.assign(1, 0);
+  for (auto& v : by_depth) {
+    levels.insert(levels.end(), v.begin(), v.end());
+    s->level_off.push_back((int64_t)levels.size());
+  }
 
   int rc;
   if ((rc = s->refs.reserve(nrec * sizeof(mh::RefNode)))) return rc;
@@ -158,21 +165,20 @@ int upload(mh_scene* s) {
   if ((rc = s->tris.reserve((size_t)T * 12 * sizeof(float)))) return rc;
   if ((rc = s->ids.reserve((size_t)T * sizeof(int32_t)))) return rc;
   if ((rc = s->big.reserve((size_t)T * sizeof(int32_t)))) return rc;
-  if ((rc = s->parent.reserve(nrec * sizeof(int32_t)))) return rc;
   if ((rc = s->slot.reserve(nrec * sizeof(int32_t)))) return rc;
   if ((rc = s->leaves.reserve(std::max<size_t>(1, leaves.size()) * sizeof(int32_t)))) return rc;
-  if ((rc = s->arrive.reserve(nrec * sizeof(uint32_t)))) return rc;
+  if ((rc = s->levels.reserve(std::max<size_t>(1, levels.size()) * sizeof(int32_t)))) return rc;
   HIP_OK(hipMemcpy(s->refs.p, nodes.data(), nrec * sizeof(mh::RefNode), hipMemcpyHostToDevice));
   if (!pairs.empty())
     HIP_OK(hipMemcpy(s->pairs.p, pairs.data(), pairs.size() * sizeof(mh::PairNode),
                      hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->ids.p, s->bvh.ids.data(), (size_t)T * 4, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->big.p, big.data(), (size_t)T * 4, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->parent.p, parent.data(), nrec * 4, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->slot.p, slot.data(), nrec * 4, hipMemcpyHostToDevice));
   if (!leaves.empty())
     HIP_OK(hipMemcpy(s->leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice));
-  HIP_OK(hipMemset(s->arrive.p, 0, nrec * 4));
+  if (!levels.empty())
+    HIP_OK(hipMemcpy(s->levels.p, levels.data(), levels.size() * 4, hipMemcpyHostToDevice));
 
   mh::DevRefit r{};
   r.tris = s->tris.as<float>();
@@ -515,14 +521,14 @@ int mh_scene_refit(mh_scene* s, void* stream) {
   r.obj_ids = s->ids.as<int32_t>();
   r.verts = s->d_verts.as<float>();
   r.idx = s->d_idx.as<uint32_t>();
-  r.parent = s->parent.as<int32_t>();
   r.slot = s->slot.as<int32_t>();
   r.leaves = s->leaves.as<int32_t>();
-  r.arrive = s->arrive.as<uint32_t>();
+  r.levels = s->levels.as<int32_t>();
   r.n_leaves = (int32_t)s->n_leaves;
   r.n_slots = (int32_t)s->nt;
   r.prim_kind = s->prim_kind;
-  HIP_OK((hipError_t)mh::launch_refit(r, stream));
+  HIP_OK((hipError_t)mh::launch_refit(r, s->level_off.data(), (int)s->level_off.size() - 1,
+                                      stream));
   return MH_OK;
 }
 

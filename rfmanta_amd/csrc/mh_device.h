@@ -102,6 +102,7 @@ struct BatchLaunch {
   int refill;     // idle lanes that trigger a refill, 1..63; 64 = whole-wave chunks
   int chunk;      // rays per counter grab (multiple of 64); 0 = static stride, no counter
   int xcd_remap;  // contiguous ray shares per XCD
+  int kinds;      // 0 mixed table, 1 closest-hit batches only, 2 any-hit batches only
 };
 
 int launch_trace_batch(const DevScene& sc, const BatchLaunch& bl, void* stream);
@@ -117,10 +118,9 @@ struct DevRefit {
   const int32_t* obj_ids;
   const float* verts;
   const uint32_t* idx;
-  const int32_t* parent;  // parent record id, -1 for the root
   const int32_t* slot;    // pair index * 2 + side for a non-root record, -1 for the root
   const int32_t* leaves;  // leaf record ids
-  uint32_t* arrive;       // per-record arrival counters (never reset: odd = second arrival)
+  const int32_t* levels;  // inner record ids grouped by depth (level_off gives the groups)
   int32_t n_leaves;
   int32_t n_slots;        // T
   int32_t prim_kind;
@@ -128,7 +128,9 @@ struct DevRefit {
 
 // refit.hip
 int launch_prim_setup(const DevRefit& r, void* stream);  // tris from verts (all slots)
-int launch_refit(const DevRefit& r, void* stream);       // tris + bounds, bottom-up
+// tris + leaf boxes, then one launch per level of inner records from the deepest up;
+// level_off (host, n_levels + 1 entries) delimits each depth's records in r.levels.
+int launch_refit(const DevRefit& r, const int64_t* level_off, int n_levels, void* stream);
 int launch_displace_sine(float* verts, const float* base, int64_t n_verts, float amp, float freq,
                          float phase, void* stream);
 

@@ -1,16 +1,17 @@
 // Per-frame refit kernels for gfx950.
 //
-// k_refit: one lane per leaf record. The lane rebuilds the triangle records of its leaf
-// from the current vertices, forms the leaf box (Mesh::getBBox + BBox::extendByBox,
-// Model/Groups/Mesh.cc:228-237, DynBVH.cc:2091-2097), stores it in the reference record
-// and in the parent's PairNode, then climbs: at each parent an agent-scope acq_rel
-// counter lets the second child to arrive form the union of both child boxes
-// (DynBVH.cc:2098-2117, updateBounds<false> without rotations) and keep climbing. The
-// counters are never cleared: each refit adds exactly two arrivals per inner record, so
-// an odd previous value identifies the second arrival.
-//
+// The refit is DynBVH::updateBounds<false> without rotations (DynBVH.cc:2084-2117), done
+// level-synchronously:
+//   k_refit_leaves: one lane per leaf record rebuilds the triangle records of its leaf
+//     from the current vertices and forms the leaf box (Mesh::getBBox +
+//     BBox::extendByBox, Model/Groups/Mesh.cc:228-237, DynBVH.cc:2091-2097);
+//   k_refit_level: one launch per depth, deepest first; each lane unions the two child
+//     boxes of one inner record (DynBVH.cc:2110-2116).
+// Every box is stored both in the reference record and in the parent's PairNode. Kernel
+// boundaries on the stream order the levels, so no cross-workgroup synchronisation is
+// needed (agent-scope acquire/release would write back and invalidate the per-XCD L2s).
 // Box arithmetic is exact (min/max of the same fp32 values), so the result is
-// bit-identical to a serial recomputation in any arrival order.
+// bit-identical to a serial recomputation.
 
 #include <hip/hip_runtime.h>
 
@@ -152,10 +153,10 @@ __device__ __forceinline__ void store_box(const DevRefit& r, int rec, const floa
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_refit(DevRefit r) {
+__global__ __launch_bounds__(kBlock) void k_refit_leaves(DevRefit r) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= r.n_leaves) return;
-  int rec = r.leaves[i];
+  const int rec = r.leaves[i];
   const RefNode leaf = r.refs[rec];
   float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   for (int k = 0; k < leaf.children; ++k) {
@@ -171,23 +172,22 @@ __global__ __launch_bounds__(kBlock) void k_refit(DevRefit r) {
     }
   }
   store_box(r, rec, lo, hi);
-  for (;;) {
-    const int p = r.parent[rec];
-    if (p < 0) break;
-    const unsigned old =
-        __hip_atomic_fetch_add(r.arrive + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if ((old & 1u) == 0u) break;  // the sibling subtree is still being refit
-    const int c = r.refs[p].child;
-    const RefNode a = r.refs[c];
-    const RefNode b = r.refs[c + 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_level(DevRefit r, int64_t first, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int rec = r.levels[first + i];
+  const int c = r.refs[rec].child;
+  const RefNode a = r.refs[c];
+  const RefNode b = r.refs[c + 1];
+  float lo[3], hi[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = pick_lo(pick_lo(FLT_MAX, a.box.lo[k]), b.box.lo[k]);
-      hi[k] = pick_hi(pick_hi(-FLT_MAX, a.box.hi[k]), b.box.hi[k]);
-    }
-    store_box(r, p, lo, hi);
-    rec = p;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = pick_lo(pick_lo(FLT_MAX, a.box.lo[k]), b.box.lo[k]);
+    hi[k] = pick_hi(pick_hi(-FLT_MAX, a.box.hi[k]), b.box.hi[k]);
   }
+  store_box(r, rec, lo, hi);
 }
 
 __global__ __launch_bounds__(kBlock) void k_displace_sine(float* verts, const float* base,
@@ -211,11 +211,17 @@ int launch_prim_setup(const DevRefit& r, void* stream) {
   return (int)hipGetLastError();
 }
 
-int launch_refit(const DevRefit& r, void* stream) {
+int launch_refit(const DevRefit& r, const int64_t* level_off, int n_levels, void* stream) {
   if (r.n_leaves <= 0) return 0;
-  const unsigned blocks = (unsigned)((r.n_leaves + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(kBlock), 0,
-                     reinterpret_cast<hipStream_t>(stream), r);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  unsigned blocks = (unsigned)((r.n_leaves + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_refit_leaves, dim3(blocks), dim3(kBlock), 0, s, r);
+  for (int d = n_levels - 1; d >= 0; --d) {
+    const int64_t first = level_off[d], n = level_off[d + 1] - level_off[d];
+    if (n <= 0) continue;
+    blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_refit_level, dim3(blocks), dim3(kBlock), 0, s, r, first, n);
+  }
   return (int)hipGetLastError();
 }
 

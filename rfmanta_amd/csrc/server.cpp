@@ -3,22 +3,23 @@
 //
 // Threads: callers submit and wait; one scheduler thread owns the device work. Each
 // scheduler round
-//   1. admits queued requests in arrival order while their buffers fit the arena (the
-//      HBM budget) and fewer than max_active are in flight, generating their camera rays
-//      on the device;
-//   2. builds, per scene, a batch table with one chunk of every admitted request's
-//      remaining rays (primary closest-hit first, then its shadow rays, generated on the
-//      device once the primary rays are done) and launches one persistent trace per
-//      scene, each scene on its own stream so that the launches overlap;
-//   3. waits for the launches, and completes every request whose rays are all traced:
-//      its hits are copied to the host and its arena range is freed.
-// A request therefore joins the device work at the first round after it fits, and large
+//   1. admits queued requests in arrival order while their device buffers fit the device
+//      arena (the HBM budget) and fewer than max_active are in flight, generating their
+//      camera rays on the device;
+//   2. builds, per scene, a batch table with one chunk of each remaining ray set of every
+//      admitted request (primary closest-hit first; then its shadow (any-hit) and bounce
+//      (closest-hit) rays, generated on the device from the primary hits) and launches
+//      one persistent trace per scene, each scene on its own stream so that the launches
+//      overlap;
+//   3. waits for the launches; a request whose ray sets are all traced gets a range of the
+//      pinned host arena, its results are copied there and its device range is freed.
+// mh_server_wait copies a finished request's results out of the host arena and frees
+// that range. A request joins the device work at the first round after it fits, so large
 // requests never hold small ones back for more than one round.
 
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -40,12 +41,17 @@ using mh_internal::fail;
 
 namespace {
 
-// First-fit range allocator over [0, size) with coalescing free list.
+constexpr size_t kAlign = 256;
+size_t up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit range allocator over [0, size) with a coalescing free list.
 class Arena {
  public:
-  explicit Arena(size_t size) { free_[0] = size; }
+  explicit Arena(size_t size) {
+    if (size) free_[0] = size;
+  }
   bool alloc(size_t n, size_t& off) {
-    n = (n + 255) & ~size_t(255);
+    n = up(n);
     for (auto it = free_.begin(); it != free_.end(); ++it) {
       if (it->second >= n) {
         off = it->first;
@@ -60,7 +66,7 @@ class Arena {
     return false;
   }
   void release(size_t off, size_t n) {
-    n = (n + 255) & ~size_t(255);
+    n = up(n);
     used_ -= n;
     auto it = free_.emplace(off, n).first;
     auto nx = std::next(it);
@@ -83,6 +89,26 @@ class Arena {
   size_t used_ = 0;
 };
 
+// Hands out 256-byte aligned sub-ranges of one allocation.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(int64_t n) {
+    T* p = reinterpret_cast<T*>(base + off);
+    off += up(sizeof(T) * (size_t)n);
+    return p;
+  }
+};
+
+// One set of rays of a request and where its hits go.
+struct RaySet {
+  mh_rays rays{};
+  mh_hits hits{};
+  int64_t next = 0;  // rays handed to launches so far
+  bool any = false;
+};
+
 struct Request {
   uint64_t id = 0;
   int scene = 0;
@@ -90,26 +116,31 @@ struct Request {
   int64_t n = 0;  // primary rays
   int status = MH_OK;
   double t_submit = 0, t_admit = 0, t_done = 0;
-  // device arena range and the arrays inside it
-  size_t off = 0, bytes = 0;
-  mh_rays prim{};
-  mh_hits hits{};
-  mh_rays shadow{};
-  uint8_t* occluded = nullptr;
-  int32_t* shadow_prim = nullptr;
-  int64_t next = 0, sh_next = 0, sh_done_rounds = 0;
-  bool primary_done = false, shadow_generated = false, done = false;
-  // host results
-  std::vector<int32_t> h_prim;
-  std::vector<float> h_t;
-  std::vector<uint8_t> h_occ;
+  size_t dev_off = 0, dev_bytes = 0;    // device arena range
+  size_t host_off = 0, host_bytes = 0;  // host arena range (results)
+  bool has_host = false;
+  RaySet prim;
+  std::vector<RaySet> sec;  // shadow and/or bounce
+  bool primary_done = false, sec_generated = false, done = false;
+  // results in the host arena
+  int32_t* h_prim = nullptr;
+  float* h_t = nullptr;
+  uint8_t* h_occ = nullptr;
+  int32_t* h_bprim = nullptr;
+  float* h_bt = nullptr;
 };
 
-size_t request_bytes(int64_t n, bool shadows) {
-  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-  size_t b = 7 * up(4 * (size_t)n) + 4 * up(4 * (size_t)n);  // primary rays + hits
-  if (shadows) b += 7 * up(4 * (size_t)n) + 2 * up((size_t)n) + up(4 * (size_t)n);
-  return b;
+size_t device_bytes(int64_t n, bool shadows, bool bounces) {
+  const size_t f = up(4 * (size_t)n), b = up((size_t)n);
+  size_t s = 7 * f + 4 * f;             // primary rays, hits (prim t u v)
+  if (shadows) s += 7 * f + 2 * b;      // shadow rays + mask, occluded
+  if (bounces) s += 7 * f + b + 2 * f;  // bounce rays + mask, hits (prim t)
+  return s;
+}
+
+size_t host_bytes(int64_t n, bool shadows, bool bounces) {
+  const size_t f = up(4 * (size_t)n), b = up((size_t)n);
+  return 2 * f + (shadows ? b : 0) + (bounces ? 2 * f : 0);
 }
 
 }  // namespace
@@ -120,6 +151,8 @@ struct mh_server {
   mh_launch_config launch{};
   DevBuf arena_mem;
   std::unique_ptr<Arena> arena;
+  char* host_mem = nullptr;
+  std::unique_ptr<Arena> host_arena;
   // per scene: stream, batch table (device + pinned host), counters, spill
   struct PerScene {
     hipStream_t stream = nullptr;
@@ -149,10 +182,12 @@ struct mh_server {
       if (p->stream) (void)hipStreamDestroy(p->stream);
       if (p->host_table) (void)hipHostFree(p->host_table);
     }
+    if (host_mem) (void)hipHostFree(host_mem);
   }
+  bool progressed = false;  // the last round launched or finished something
   void run();
   int round();
-  int admit_locked(std::vector<std::shared_ptr<Request>>& fresh);
+  void admit_locked(std::vector<std::shared_ptr<Request>>& fresh);
 };
 
 namespace {
@@ -161,54 +196,77 @@ constexpr int kTableCap = MH_MAX_BATCHES;
 
 size_t table_bytes() { return offsetof(mh::BatchTable, batches) + sizeof(mh::BatchDev) * kTableCap; }
 
-mh::BatchDev chunk(const mh_rays& r, const mh_hits& h, int64_t first, int64_t count, bool any) {
-  auto off = [&](const float* p) { return p ? p + first : nullptr; };
+mh::BatchDev chunk(const RaySet& s, int64_t count) {
+  const int64_t first = s.next;
+  auto rd = [&](const float* p) { return p ? p + first : nullptr; };
+  auto wr = [&](float* p) { return p ? p + first : nullptr; };
+  const mh_rays& r = s.rays;
+  const mh_hits& h = s.hits;
   mh::BatchDev d{};
-  d.rays = mh::DevRays{off(r.ox), off(r.oy), off(r.oz), off(r.dx), off(r.dy), off(r.dz),
-                       off(r.tmax), r.masked ? r.masked + first : nullptr};
-  auto offw = [&](float* p) { return p ? p + first : nullptr; };
-  d.hits = mh::DevHits{h.prim ? h.prim + first : nullptr, offw(h.t), offw(h.u), offw(h.v),
+  d.rays = mh::DevRays{rd(r.ox), rd(r.oy), rd(r.oz), rd(r.dx), rd(r.dy), rd(r.dz), rd(r.tmax),
+                       r.masked ? r.masked + first : nullptr};
+  d.hits = mh::DevHits{h.prim ? h.prim + first : nullptr, wr(h.t), wr(h.u), wr(h.v),
                        h.occluded ? h.occluded + first : nullptr};
   d.count = count;
-  d.flags = any ? 1u : 0u;
+  d.flags = s.any ? 1u : 0u;
   return d;
+}
+
+mh::RayOut ray_out(const mh_rays& r) {
+  return mh::RayOut{const_cast<float*>(r.ox), const_cast<float*>(r.oy), const_cast<float*>(r.oz),
+                    const_cast<float*>(r.dx), const_cast<float*>(r.dy), const_cast<float*>(r.dz),
+                    const_cast<float*>(r.tmax), const_cast<uint8_t*>(r.masked)};
 }
 
 }  // namespace
 
-int mh_server::admit_locked(std::vector<std::shared_ptr<Request>>& fresh) {
+void mh_server::admit_locked(std::vector<std::shared_ptr<Request>>& fresh) {
+  const bool sh = opts.shadows != 0, bo = opts.bounces != 0;
   while (!pending.empty() && (int)active.size() < opts.max_active) {
     std::shared_ptr<Request> r = pending.front();
-    const size_t need = request_bytes(r->n, opts.shadows != 0);
+    const size_t need = device_bytes(r->n, sh, bo);
     size_t off;
     if (!arena->alloc(need, off)) break;  // wait for memory to free up (arrival order)
     pending.pop_front();
-    r->off = off;
-    r->bytes = need;
+    r->dev_off = off;
+    r->dev_bytes = need;
     r->t_admit = now();
-    char* base = arena_mem.as<char>() + off;
-    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t f = up(4 * (size_t)r->n), b1 = up((size_t)r->n);
-    float* a[7];
-    for (int i = 0; i < 7; ++i) a[i] = reinterpret_cast<float*>(base + i * f);
-    r->prim = mh_rays{a[0], a[1], a[2], a[3], a[4], a[5], a[6], nullptr};
-    char* hb = base + 7 * f;
-    r->hits = mh_hits{reinterpret_cast<int32_t*>(hb), reinterpret_cast<float*>(hb + f),
-                      reinterpret_cast<float*>(hb + 2 * f), reinterpret_cast<float*>(hb + 3 * f),
-                      nullptr};
-    if (opts.shadows) {
-      char* sb = hb + 4 * f;
-      float* s[7];
-      for (int i = 0; i < 7; ++i) s[i] = reinterpret_cast<float*>(sb + i * f);
-      uint8_t* smask = reinterpret_cast<uint8_t*>(sb + 7 * f);
-      r->occluded = reinterpret_cast<uint8_t*>(sb + 7 * f + b1);
-      r->shadow_prim = reinterpret_cast<int32_t*>(sb + 7 * f + 2 * b1);
-      r->shadow = mh_rays{s[0], s[1], s[2], s[3], s[4], s[5], s[6], smask};
+    Carver c{arena_mem.as<char>() + off};
+    const int64_t n = r->n;
+    auto rays = [&](bool masked) {
+      mh_rays x{};
+      x.ox = c.take<float>(n);
+      x.oy = c.take<float>(n);
+      x.oz = c.take<float>(n);
+      x.dx = c.take<float>(n);
+      x.dy = c.take<float>(n);
+      x.dz = c.take<float>(n);
+      x.tmax = c.take<float>(n);
+      x.masked = masked ? c.take<uint8_t>(n) : nullptr;
+      return x;
+    };
+    r->prim.rays = rays(false);
+    r->prim.hits.prim = c.take<int32_t>(n);
+    r->prim.hits.t = c.take<float>(n);
+    r->prim.hits.u = c.take<float>(n);
+    r->prim.hits.v = c.take<float>(n);
+    if (sh) {
+      RaySet s;
+      s.rays = rays(true);
+      s.hits.occluded = c.take<uint8_t>(n);
+      s.any = true;
+      r->sec.push_back(s);
+    }
+    if (bo) {
+      RaySet b;
+      b.rays = rays(true);
+      b.hits.prim = c.take<int32_t>(n);
+      b.hits.t = c.take<float>(n);
+      r->sec.push_back(b);
     }
     active.push_back(r);
     fresh.push_back(r);
   }
-  return MH_OK;
 }
 
 int mh_server::round() {
@@ -226,38 +284,30 @@ int mh_server::round() {
       r->status = rc;
       continue;
     }
-    mh::RayOut o{const_cast<float*>(r->prim.ox), const_cast<float*>(r->prim.oy),
-                 const_cast<float*>(r->prim.oz), const_cast<float*>(r->prim.dx),
-                 const_cast<float*>(r->prim.dy), const_cast<float*>(r->prim.dz),
-                 const_cast<float*>(r->prim.tmax), nullptr};
-    HIP_OK((hipError_t)mh::launch_camera(cd, o, per[r->scene]->stream));
+    HIP_OK((hipError_t)mh::launch_camera(cd, ray_out(r->prim.rays), per[r->scene]->stream));
   }
-  // One batch table per scene: a chunk of each request's remaining rays.
+  // One batch table per scene: a chunk of each remaining ray set of each request.
   const int ns = (int)scenes.size();
-  std::vector<int> nb(ns, 0);
+  std::vector<int> nb(ns, 0), n_any(ns, 0);
   std::vector<int64_t> total(ns, 0);
-  std::vector<std::vector<std::pair<Request*, int64_t>>> plan(ns);  // (request, rays scheduled)
+  auto add = [&](Request& r, RaySet& s) {
+    const int k = r.scene;
+    if (s.next >= r.n || nb[k] >= kTableCap) return;
+    const int64_t c = std::min<int64_t>(opts.chunk_rays, r.n - s.next);
+    mh::BatchTable* t = reinterpret_cast<mh::BatchTable*>(per[k]->host_table);
+    mh::BatchDev d = chunk(s, c);
+    d.start = total[k];
+    t->batches[nb[k]++] = d;
+    n_any[k] += s.any ? 1 : 0;
+    total[k] += c;
+    s.next += c;
+  };
   for (auto& r : work) {
     if (r->status != MH_OK || r->done) continue;
-    const int k = r->scene;
-    mh::BatchTable* t = reinterpret_cast<mh::BatchTable*>(per[k]->host_table);
-    if (nb[k] >= kTableCap) continue;
-    if (!r->primary_done) {
-      const int64_t c = std::min<int64_t>(opts.chunk_rays, r->n - r->next);
-      mh::BatchDev d = chunk(r->prim, r->hits, r->next, c, false);
-      d.start = total[k];
-      t->batches[nb[k]++] = d;
-      total[k] += c;
-      r->next += c;
-    } else if (opts.shadows && r->shadow_generated) {
-      const int64_t c = std::min<int64_t>(opts.chunk_rays, r->n - r->sh_next);
-      mh_hits sh{r->shadow_prim, nullptr, nullptr, nullptr, r->occluded};
-      mh::BatchDev d = chunk(r->shadow, sh, r->sh_next, c, true);
-      d.start = total[k];
-      t->batches[nb[k]++] = d;
-      total[k] += c;
-      r->sh_next += c;
-    }
+    if (!r->primary_done)
+      add(*r, r->prim);
+    else if (r->sec_generated)
+      for (auto& s : r->sec) add(*r, s);
   }
   for (int k = 0; k < ns; ++k) {
     if (nb[k] == 0) continue;
@@ -279,12 +329,13 @@ int mh_server::round() {
     bl.refill = launch.refill;
     bl.chunk = launch.chunk;
     bl.xcd_remap = launch.xcd_remap;
+    bl.kinds = n_any[k] == 0 ? 1 : (n_any[k] == nb[k] ? 2 : 0);
     HIP_OK((hipError_t)mh::launch_trace_batch(scenes[k]->dev(), bl, per[k]->stream));
   }
   for (int k = 0; k < ns; ++k) HIP_OK(hipStreamSynchronize(per[k]->stream));
   int64_t traced = 0;
   for (int k = 0; k < ns; ++k) traced += total[k];
-  // Progress: primary done -> shadow generation; everything done -> results to host.
+  // Progress: primary done -> secondary generation; all done -> results to the host.
   std::vector<std::shared_ptr<Request>> completed;
   for (auto& r : work) {
     if (r->done) continue;
@@ -292,41 +343,65 @@ int mh_server::round() {
       completed.push_back(r);
       continue;
     }
-    if (!r->primary_done && r->next >= r->n) {
+    if (!r->primary_done && r->prim.next >= r->n) {
       r->primary_done = true;
-      if (opts.shadows) {
-        mh::SecondaryArgs a{};
-        a.in = mh::DevRays{r->prim.ox, r->prim.oy, r->prim.oz, r->prim.dx, r->prim.dy, r->prim.dz,
-                           r->prim.tmax, nullptr};
-        a.hit_prim = r->hits.prim;
-        a.hit_t = r->hits.t;
-        a.verts = scenes[r->scene]->d_verts.as<const float>();
-        a.idx = scenes[r->scene]->d_idx.as<const uint32_t>();
-        for (int i = 0; i < 3; ++i) a.light[i] = opts.light[i];
-        a.n = r->n;
-        a.out = mh::RayOut{const_cast<float*>(r->shadow.ox), const_cast<float*>(r->shadow.oy),
-                           const_cast<float*>(r->shadow.oz), const_cast<float*>(r->shadow.dx),
-                           const_cast<float*>(r->shadow.dy), const_cast<float*>(r->shadow.dz),
-                           const_cast<float*>(r->shadow.tmax),
-                           const_cast<uint8_t*>(r->shadow.masked)};
-        HIP_OK((hipError_t)mh::launch_secondary(a, 0, per[r->scene]->stream));
-        r->shadow_generated = true;
+      const mh_scene* sc = scenes[r->scene];
+      mh::SecondaryArgs a{};
+      const mh_rays& p = r->prim.rays;
+      a.in = mh::DevRays{p.ox, p.oy, p.oz, p.dx, p.dy, p.dz, p.tmax, nullptr};
+      a.hit_prim = r->prim.hits.prim;
+      a.hit_t = r->prim.hits.t;
+      a.verts = sc->d_verts.as<const float>();
+      a.idx = sc->d_idx.as<const uint32_t>();
+      for (int i = 0; i < 3; ++i) a.light[i] = opts.light[i];
+      a.seed = r->cam.seed + 1;
+      a.n = r->n;
+      for (auto& s : r->sec) {
+        a.out = ray_out(s.rays);
+        HIP_OK((hipError_t)mh::launch_secondary(a, s.any ? 0 : 1, per[r->scene]->stream));
       }
+      r->sec_generated = true;
     }
-    const bool all = r->primary_done && (!opts.shadows || (r->shadow_generated && r->sh_next >= r->n));
+    bool all = r->primary_done;
+    for (auto& s : r->sec) all = all && s.next >= r->n;
     if (all) completed.push_back(r);
   }
-  for (auto& r : completed) {
-    if (r->status == MH_OK) {
-      hipStream_t s = per[r->scene]->stream;
-      r->h_prim.resize(r->n);
-      r->h_t.resize(r->n);
-      HIP_OK(hipMemcpyAsync(r->h_prim.data(), r->hits.prim, 4 * (size_t)r->n,
-                            hipMemcpyDeviceToHost, s));
-      HIP_OK(hipMemcpyAsync(r->h_t.data(), r->hits.t, 4 * (size_t)r->n, hipMemcpyDeviceToHost, s));
-      if (opts.shadows) {
-        r->h_occ.resize(r->n);
-        HIP_OK(hipMemcpyAsync(r->h_occ.data(), r->occluded, (size_t)r->n, hipMemcpyDeviceToHost, s));
+  // Host ranges for the finished requests (those that do not fit wait for collection).
+  std::vector<std::shared_ptr<Request>> copied;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& r : completed) {
+      if (r->status != MH_OK) {
+        copied.push_back(r);
+        continue;
+      }
+      const size_t hb = host_bytes(r->n, opts.shadows != 0, opts.bounces != 0);
+      size_t off;
+      if (!host_arena->alloc(hb, off)) continue;
+      r->host_off = off;
+      r->host_bytes = hb;
+      r->has_host = true;
+      copied.push_back(r);
+    }
+  }
+  for (auto& r : copied) {
+    if (r->status != MH_OK) continue;
+    hipStream_t s = per[r->scene]->stream;
+    const int64_t n = r->n;
+    Carver c{host_mem + r->host_off};
+    r->h_prim = c.take<int32_t>(n);
+    r->h_t = c.take<float>(n);
+    HIP_OK(hipMemcpyAsync(r->h_prim, r->prim.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(r->h_t, r->prim.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    for (auto& x : r->sec) {
+      if (x.any) {
+        r->h_occ = c.take<uint8_t>(n);
+        HIP_OK(hipMemcpyAsync(r->h_occ, x.hits.occluded, (size_t)n, hipMemcpyDeviceToHost, s));
+      } else {
+        r->h_bprim = c.take<int32_t>(n);
+        r->h_bt = c.take<float>(n);
+        HIP_OK(hipMemcpyAsync(r->h_bprim, x.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(r->h_bt, x.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
       }
     }
   }
@@ -334,21 +409,21 @@ int mh_server::round() {
   {
     std::lock_guard<std::mutex> lk(mu);
     st.rays += traced;
-    for (auto& r : completed) {
+    for (auto& r : copied) {
       r->t_done = now();
       r->done = true;
-      arena->release(r->off, r->bytes);
+      arena->release(r->dev_off, r->dev_bytes);
       active.erase(std::find(active.begin(), active.end(), r));
       finished[r->id] = r;
       st.completed++;
     }
   }
-  if (!completed.empty()) cv_done.notify_all();
+  progressed = !fresh.empty() || traced > 0 || !copied.empty();
+  if (!copied.empty()) cv_done.notify_all();
   return MH_OK;
 }
 
 void mh_server::run() {
-  (void)hipSetDevice(0);
   (void)ensure_init();
   for (;;) {
     {
@@ -358,7 +433,7 @@ void mh_server::run() {
     }
     const double t_start = now();
     const int rc = round();
-    std::lock_guard<std::mutex> lk(mu);
+    std::unique_lock<std::mutex> lk(mu);
     st.busy_ms += now() - t_start;
     if (rc != MH_OK) {
       // A device error ends every request in flight with that status.
@@ -366,11 +441,32 @@ void mh_server::run() {
         r->status = rc;
         r->done = true;
         r->t_done = now();
-        arena->release(r->off, r->bytes);
+        arena->release(r->dev_off, r->dev_bytes);
+        if (r->has_host) host_arena->release(r->host_off, r->host_bytes);
+        r->has_host = false;
         finished[r->id] = r;
       }
       active.clear();
       cv_done.notify_all();
+    } else if (!progressed) {
+      // Finished requests that found no host space wait for collections
+      // (mh_server_wait notifies) instead of spinning.
+      cv_work.wait_for(lk, std::chrono::milliseconds(1));
+    }
+    if (stop) {
+      // Uncollected results are discarded at shutdown so the drain cannot stall on them.
+      for (auto& kv : finished) {
+        Request& r = *kv.second;
+        if (!r.has_host) continue;
+        host_arena->release(r.host_off, r.host_bytes);
+        r.has_host = false;
+        r.h_prim = nullptr;
+        r.h_t = nullptr;
+        r.h_occ = nullptr;
+        r.h_bprim = nullptr;
+        r.h_bt = nullptr;
+        if (r.status == MH_OK) r.status = MH_E_STATE;
+      }
     }
   }
 }
@@ -389,23 +485,21 @@ int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_serve
   if (!s) return fail(MH_E_NOMEM, "server");
   s->scenes.assign(scenes, scenes + n_scenes);
   mh_server_opts d{};
-  d.hbm_budget = size_t(1) << 30;
-  d.chunk_rays = int64_t(1) << 20;
-  d.max_active = 64;
-  d.shadows = 0;
-  if (o) {
-    d = *o;
-    if (!d.hbm_budget) d.hbm_budget = size_t(1) << 30;
-    if (d.chunk_rays <= 0) d.chunk_rays = int64_t(1) << 20;
-    if (d.max_active <= 0) d.max_active = 64;
-  }
+  if (o) d = *o;
+  if (!d.hbm_budget) d.hbm_budget = size_t(1) << 30;
+  if (!d.host_budget) d.host_budget = d.hbm_budget / 4;
+  if (d.chunk_rays <= 0) d.chunk_rays = int64_t(1) << 20;
+  if (d.max_active <= 0) d.max_active = 64;
+  if (d.max_active > kTableCap / 2) d.max_active = kTableCap / 2;  // two table entries each
   s->opts = d;
-  s->launch = (d.launch.block == 0) ? mh_launch_config{MH_LAYOUT_PAIR, 8, 256, 8, 64, 256, 1}
+  s->launch = (d.launch.block == 0) ? mh_launch_config{MH_LAYOUT_PAIR, 8, 256, 8, 64, 256, 0}
                                     : d.launch;
-  if (s->opts.max_active > kTableCap) s->opts.max_active = kTableCap;
   if ((rc = mh_internal::check_launch_config(s->launch))) return rc;
   if ((rc = s->arena_mem.reserve(d.hbm_budget))) return rc;
   s->arena.reset(new Arena(d.hbm_budget));
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&s->host_mem), d.host_budget,
+                       hipHostMallocDefault));
+  s->host_arena.reset(new Arena(d.host_budget));
   for (int k = 0; k < n_scenes; ++k) s->per.emplace_back(new mh_server::PerScene());
   for (int k = 0; k < n_scenes; ++k) {
     auto& p = *s->per[k];
@@ -436,12 +530,14 @@ int mh_server_submit(mh_server* s, const mh_request* req, uint64_t* id) {
   r->scene = req->scene;
   r->cam = req->camera;
   r->n = (int64_t)req->camera.xres * req->camera.yres * req->camera.spp;
-  const size_t need = request_bytes(r->n, s->opts.shadows != 0);
+  const bool sh = s->opts.shadows != 0, bo = s->opts.bounces != 0;
+  const size_t need = device_bytes(r->n, sh, bo), hneed = host_bytes(r->n, sh, bo);
   std::lock_guard<std::mutex> lk(s->mu);
-  if (need > s->opts.hbm_budget) {
+  if (need > s->opts.hbm_budget || hneed > s->opts.host_budget) {
     s->st.rejected++;
-    return fail(MH_E_UNSUPPORTED, "request needs %zu bytes, budget is %zu", need,
-                s->opts.hbm_budget);
+    return fail(MH_E_UNSUPPORTED,
+                "request needs %zu device / %zu host bytes, budgets are %zu / %zu", need, hneed,
+                s->opts.hbm_budget, s->opts.host_budget);
   }
   r->id = s->next_id++;
   r->t_submit = s->now();
@@ -452,7 +548,7 @@ int mh_server_submit(mh_server* s, const mh_request* req, uint64_t* id) {
   return MH_OK;
 }
 
-int mh_server_wait(mh_server* s, uint64_t id, int32_t* prim, float* t, uint8_t* occluded,
+int mh_server_wait(mh_server* s, uint64_t id, const mh_request_result* out,
                    mh_request_info* info) {
   if (!s) return fail(MH_E_INVALID, "null server");
   std::shared_ptr<Request> r;
@@ -477,15 +573,26 @@ int mh_server_wait(mh_server* s, uint64_t id, int32_t* prim, float* t, uint8_t* 
   if (info) {
     info->id = r->id;
     info->status = r->status;
-    info->rays = r->n * (s->opts.shadows ? 2 : 1);
+    info->rays = r->n * (1 + (int64_t)r->sec.size());
     info->submit_ms = r->t_submit;
     info->admit_ms = r->t_admit;
     info->done_ms = r->t_done;
   }
+  if (r->status == MH_OK && out) {
+    const size_t n = (size_t)r->n;
+    if (out->prim && r->h_prim) std::memcpy(out->prim, r->h_prim, 4 * n);
+    if (out->t && r->h_t) std::memcpy(out->t, r->h_t, 4 * n);
+    if (out->occluded && r->h_occ) std::memcpy(out->occluded, r->h_occ, n);
+    if (out->bounce_prim && r->h_bprim) std::memcpy(out->bounce_prim, r->h_bprim, 4 * n);
+    if (out->bounce_t && r->h_bt) std::memcpy(out->bounce_t, r->h_bt, 4 * n);
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->has_host) s->host_arena->release(r->host_off, r->host_bytes);
+    r->has_host = false;
+  }
+  s->cv_work.notify_one();
   if (r->status != MH_OK) return fail(r->status, "request %llu failed", (unsigned long long)id);
-  if (prim) std::memcpy(prim, r->h_prim.data(), 4 * (size_t)r->n);
-  if (t) std::memcpy(t, r->h_t.data(), 4 * (size_t)r->n);
-  if (occluded && !r->h_occ.empty()) std::memcpy(occluded, r->h_occ.data(), (size_t)r->n);
   return MH_OK;
 }
 
@@ -496,6 +603,7 @@ int mh_server_get_stats(mh_server* s, mh_server_stats* out) {
   out->active = (int)s->active.size();
   out->pending = (int)s->pending.size();
   out->hbm_in_use = s->arena->used();
+  out->host_in_use = s->host_arena->used();
   return MH_OK;
 }
 

@@ -1,9 +1,10 @@
 """Serving loop (mh_server_*): results identical to the direct path, alone and in a mix.
 
 Each request's hits are checked against the same camera traced through mh_gen_camera_rays
-+ mh_trace (+ mh_gen_shadow_rays + an any-hit mh_trace), and the first request against
-the oracle. Admission is checked against the HBM budget (queued, never over budget;
-oversized requests rejected).
++ mh_trace (+ mh_gen_shadow_rays + an any-hit mh_trace, + mh_gen_bounce_rays with seed
+camera.seed + 1 + mh_trace), and the first request against the oracle. Admission is
+checked against the HBM budget (queued, never over budget; oversized requests rejected)
+and against a host staging budget that only frees up as results are collected.
 """
 
 import numpy as np
@@ -17,7 +18,7 @@ from rfmanta_amd import api, scenes  # noqa: E402
 LIGHT = (2.0, 2.0, -2.0)
 
 
-def direct(scene, cam, shadows):
+def direct(scene, cam, shadows, bounces=False):
     n = cam.xres * cam.yres * cam.spp
     dp = api.DeviceRays.alloc(n, masked=False)
     api.gen_camera_rays(cam, dp)
@@ -31,6 +32,14 @@ def direct(scene, cam, shadows):
         scene.trace(sh, so, any_hit=True)
         api.sync()
         out["occluded"] = so.download()["occluded"]
+    if bounces:
+        bo = api.DeviceRays.alloc(n)
+        api.gen_bounce_rays(scene, dp, dh, cam.seed + 1, bo)
+        bh = api.DeviceHits.alloc(n)
+        scene.trace(bo, bh)
+        api.sync()
+        b = bh.download()
+        out["bounce_prim"], out["bounce_t"] = b["prim"], b["t"]
     api.sync()
     h = dh.download()
     out["prim"], out["t"] = h["prim"], h["t"]
@@ -57,31 +66,34 @@ def cams():
     return c
 
 
-@pytest.mark.parametrize("shadows", [False, True])
-def test_alone_and_in_mix_identical(two_scenes, shadows, oracle):
+@pytest.mark.parametrize("shadows,bounces", [(False, False), (True, False), (True, True)])
+def test_alone_and_in_mix_identical(two_scenes, shadows, bounces, oracle):
     (s0, v0, i0), (s1, v1, i1) = two_scenes
     sc = [s0, s1]
     c = cams()
     jobs = [(0, c[0]), (0, c[1]), (1, c[2]), (1, c[3]), (0, c[3]), (1, c[0])]
-    ref = [direct(sc[k], cam, shadows) for k, cam in jobs]
+    ref = [direct(sc[k], cam, shadows, bounces) for k, cam in jobs]
+    keys = ("prim", "t") + (("occluded",) if shadows else ()) + \
+        (("bounce_prim", "bounce_t") if bounces else ())
     # alone: one request at a time
     with_alone = []
-    srv = api.Server(sc, shadows=shadows, light=LIGHT, chunk_rays=4096)
+    srv = api.Server(sc, shadows=shadows, bounces=bounces, light=LIGHT, chunk_rays=4096)
     for k, cam in jobs:
         with_alone.append(srv.wait(srv.submit(k, cam)))
     srv.close()
     # in a mix: all submitted at once, small chunks so requests share several rounds
-    srv = api.Server(sc, shadows=shadows, light=LIGHT, chunk_rays=3000, max_active=4)
+    srv = api.Server(sc, shadows=shadows, bounces=bounces, light=LIGHT, chunk_rays=3000,
+                     max_active=4)
     ids = [srv.submit(k, cam) for k, cam in jobs]
     mixed = [srv.wait(i) for i in reversed(ids)][::-1]
     st = srv.stats()
     srv.close()
     assert st["completed"] == len(jobs) and st["active"] == 0 and st["pending"] == 0
     for j, (r, a, m) in enumerate(zip(ref, with_alone, mixed)):
-        for key in ("prim", "t") + (("occluded",) if shadows else ()):
+        for key in keys:
             assert np.array_equal(a[key], r[key]), (j, key, "alone")
             assert np.array_equal(m[key], r[key]), (j, key, "mix")
-        assert m["status"] == 0 and m["rays"] == len(r["prim"]) * (2 if shadows else 1)
+        assert m["status"] == 0 and m["rays"] == len(r["prim"]) * (1 + shadows + bounces)
         assert m["submit_ms"] <= m["admit_ms"] <= m["done_ms"]
     # anchor the first request on the oracle
     nodes, ids_, _ = oracle.build_bvh(v0, i0)
@@ -119,4 +131,24 @@ def test_admission_respects_budget(two_scenes):
     st = srv.stats()
     assert st["rejected"] == 1 and st["completed"] == 6 and st["hbm_in_use"] == 0
     assert st["rays"] == 6 * n
+    srv.close()
+
+
+def test_host_staging_frees_on_collection(two_scenes):
+    """Results wait in pinned host staging; with room for one request's results, the others
+    finish on the device and complete one by one as results are collected."""
+    (s0, _, _), _ = two_scenes
+    cam = api.camera((0, 0.9, -1.1), (0, 0, 0), (0, 1, 0), 50.0, 64, 64)
+    n = 64 * 64
+    ref = direct(s0, cam, True, True)
+    host_one = 2 * 4 * n + n + 2 * 4 * n  # prim t, occluded, bounce prim t (256-aligned)
+    srv = api.Server([s0], shadows=True, bounces=True, light=LIGHT, host_budget=host_one)
+    ids = [srv.submit(0, cam) for _ in range(4)]
+    for i in ids:
+        o = srv.wait(i)
+        for key in ("prim", "t", "occluded", "bounce_prim", "bounce_t"):
+            assert np.array_equal(o[key], ref[key]), key
+        assert srv.stats()["host_in_use"] <= host_one
+    st = srv.stats()
+    assert st["completed"] == 4 and st["host_in_use"] == 0 and st["hbm_in_use"] == 0
     srv.close()
