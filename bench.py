@@ -70,7 +70,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--skip", default="", help="comma list of sections to skip: c2,c3,c4")
+    ap.add_argument("--skip", default="",
+                    help="comma list of sections to skip: singles (per-set frames), c2, c3, c4")
+    ap.add_argument("--c1-config", default="",
+                    help="JSON launch configuration for the C1 frame instead of autotuning")
     ap.add_argument("--c4-frames", type=int, default=100)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration)")
@@ -170,9 +173,14 @@ def main() -> int:
 
     # ---------------------------------------------------------------- batched C1 frame
     frame = api.Frame(scene, [(d_prim, h_prim, False), (d_b, h_b, False)])
-    t0 = time.time()
-    tuned, tuned_ms = frame.autotune(iters=3, cache_path="/tmp/mh_tune.json", force=True)
-    log(f"C1 frame autotune {time.time() - t0:.1f}s: {tuned} {tuned_ms:.3f} ms")
+    if args.c1_config:
+        frame.set_config(json.loads(args.c1_config))
+        tuned = frame.config()
+        log(f"C1 frame configuration {tuned}")
+    else:
+        t0 = time.time()
+        tuned, tuned_ms = frame.autotune(iters=3, cache_path="/tmp/mh_tune.json", force=True)
+        log(f"C1 frame autotune {time.time() - t0:.1f}s: {tuned} {tuned_ms:.3f} ms")
     for _ in range(args.warmup):
         frame.launch()
     api.sync()
@@ -187,6 +195,9 @@ def main() -> int:
     for name, dr, dh, anyh, n in (("coherent", d_prim, h_prim, False, n_prim),
                                   ("incoherent", d_b, h_b, False, n_b),
                                   ("shadow_anyhit", d_s, h_s, True, n_s)):
+        if "singles" in skip:
+            single[name] = {"mrays": plain[f"{name}_mrays"], "config": "plain"}
+            continue
         f1 = api.Frame(scene, [(dr, dh, anyh)])
         cfg1, ms1 = f1.autotune(iters=3, cache_path="/tmp/mh_tune.json", force=True)
         f1.launch()
