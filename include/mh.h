@@ -83,6 +83,12 @@ typedef struct mh_hits {
   uint8_t* occluded;
 } mh_hits;
 
+/* Node layouts of the single-ray traversal. All give DynBVH::intersect's one-ray results
+ * and visit order. */
+#define MH_LAYOUT_PAIR 0 /* 64-byte nodes holding both child boxes */
+#define MH_LAYOUT_REF 1  /* the reference's 32-byte BVHNode records (frames only) */
+#define MH_LAYOUT_QUAD 2 /* 128-byte nodes holding the four grandchild boxes */
+
 /* Trace options; NULL means the defaults given in brackets. */
 typedef struct mh_trace_opts {
   /* [1] 1 = each ray traverses alone (DynBVH::intersect on a one-ray packet);
@@ -93,8 +99,11 @@ typedef struct mh_trace_opts {
   int stack_lds;
   /* [0] persistent blocks per CU (0 = autotuned). */
   int blocks_per_cu;
-  /* [NULL] optional device int64[2]: box tests, triangle tests (accumulated). */
+  /* [NULL] optional device int64[2]: box tests, triangle tests (accumulated) of the
+   * binary traversal (counting runs use the PAIR layout). */
   int64_t* counters;
+  /* [MH_LAYOUT_PAIR] single-ray node layout: MH_LAYOUT_PAIR or MH_LAYOUT_QUAD. */
+  int layout;
 } mh_trace_opts;
 
 /* Build options. */
@@ -176,8 +185,6 @@ int mh_trace_host(const mh_scene* scene, const mh_rays* rays, const mh_hits* hit
  * from a shared ray counter. Per-ray results equal mh_trace with packet_size 1
  * (DynBVH::intersect on one-ray packets). A frame can be captured once into a HIP graph
  * and replayed; the batch table may be rewritten between replays (same batch count). */
-#define MH_LAYOUT_PAIR 0 /* 64-byte nodes holding both child boxes */
-#define MH_LAYOUT_REF 1  /* the reference's 32-byte BVHNode records */
 #define MH_MAX_BATCHES 256
 
 typedef struct mh_batch {
@@ -192,8 +199,9 @@ typedef struct mh_launch_config {
   int stack_lds;     /* per-lane LDS stack ring: 8, 16 or 32 entries */
   int block;         /* lanes per workgroup: 64, 128 or 256 */
   int blocks_per_cu; /* persistent workgroups per compute unit */
-  int refill;        /* idle lanes of a wave that trigger a refill, 1..63; 64 = whole-wave
-                        chunks (a wave traces `chunk` consecutive rays 64 at a time) */
+  int refill;        /* idle lanes of a wave that trigger a refill, 1..63 (PAIR and REF);
+                        64 = whole-wave chunks (a wave traces `chunk` consecutive rays 64
+                        at a time) */
   int chunk;         /* rays a wave takes per grab from the shared counter, a multiple of 64
                         (its first grab is assigned statically); 0 = static stride over the
                         rays with no counter (whole-wave mode only) */

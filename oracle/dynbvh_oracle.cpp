@@ -948,6 +948,410 @@ void parallelFor(int64_t n, int nthreads, F f) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Refit with tree rotations and leaf splits: DynBVH::update with TREE_ROT set
+// (Model/Groups/DynBVH.h:377), numProcs == 1: parallelUpdateBounds (DynBVH.cc:2012-2076)
+// -> updateBounds<false> (DynBVH.cc:2078-2123) -> rotateNode (DynBVH.cc:3051-3372), which
+// splits leaf children with the approximate SAH build (DynBVH.cc:1801-1857 with
+// useApproximateBuild, partitionApproxSAH DynBVH.cc:2125-2275, partitionObjects
+// DynBVH.cc:1369-1396, partition2Objs DynBVH.cc:2277-2308). Real is float
+// (include/CMakeLists.txt:9); BBox::computeArea's double is narrowed where the
+// reference assigns it to Real.
+// ---------------------------------------------------------------------------
+struct Updater {
+  const Mesh& mesh;
+  Node32* nodes;
+  int32_t* ids;
+  float* costs;
+  uint32_t* ssize;
+  int& next_free;
+  uint32_t large_subtree;
+  std::vector<Box> obj_bounds;
+  std::vector<Vec> obj_centroids;
+
+  Updater(const Mesh& m, Node32* n, int32_t* i, float* c, uint32_t* z, int& nf, uint32_t large)
+      : mesh(m), nodes(n), ids(i), costs(c), ssize(z), next_free(nf), large_subtree(large) {}
+
+  struct Partition {
+    Box bounds_l, bounds_r;
+    int axis;
+    int split;
+  };
+
+  static int longestAxis(const Box& b) {  // BBox.h:123-129
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    if (dx > dy) return dx > dz ? 0 : 2;
+    return dy > dz ? 1 : 2;
+  }
+  static int indexOfMax(float a, float b, float c) {  // Vector.h:382-393
+    if (a > b) return a > c ? 0 : 2;
+    return b > c ? 1 : 2;
+  }
+
+  // DynBVH.cc:2277-2308
+  Partition partition2(int nodeID, int objBegin) {
+    Partition p;
+    p.bounds_l.reset();
+    p.bounds_r.reset();
+    const float nodeArea = (float)nodes[nodeID].bounds.area();
+    const float leftArea = (float)obj_bounds[ids[objBegin]].area();
+    const float rightArea = (float)obj_bounds[ids[objBegin + 1]].area();
+    const float leafCost = kCIsec * 2;
+    const float newCost = kCTrav + (leftArea * kCIsec + rightArea * kCIsec) / nodeArea;
+    if (newCost < leafCost) {
+      const Box& nb = nodes[nodeID].bounds;
+      const int axis = indexOfMax(nb.hi[0] - nb.lo[0], nb.hi[1] - nb.lo[1], nb.hi[2] - nb.lo[2]);
+      p.axis = axis;
+      p.split = objBegin + 1;
+      if (obj_bounds[ids[objBegin]].lo[axis] > obj_bounds[ids[objBegin + 1]].lo[axis])
+        std::swap(ids[objBegin], ids[objBegin + 1]);
+      p.bounds_l = obj_bounds[ids[objBegin]];
+      p.bounds_r = obj_bounds[ids[objBegin + 1]];
+      return p;
+    }
+    p.axis = -1;
+    return p;
+  }
+
+  // DynBVH.cc:1369-1396
+  int partitionObjects(int objBegin, int objEnd, int axis, float position) {
+    int first = objBegin - 1, last = objEnd;
+    for (;;) {
+      for (++first; first < last; first++)
+        if (obj_centroids[ids[first]].v[axis] >= position) break;
+      for (--last; first < last; --last)
+        if (obj_centroids[ids[last]].v[axis] < position) break;
+      if (first < last)
+        std::swap(ids[first], ids[last]);
+      else
+        return first;
+    }
+  }
+
+  // DynBVH.cc:2125-2275 (LONGEST_AXIS is defined, so only the centroid box's longest axis)
+  Partition partitionApprox(int nodeID, int objBegin, int objEnd) {
+    Partition p;
+    p.bounds_l.reset();
+    p.bounds_r.reset();
+    const int n = objEnd - objBegin;
+    if (n == 1) {
+      p.axis = -1;
+      p.split = -1;
+      return p;
+    }
+    if (n == 2) return partition2(nodeID, objBegin);
+    Box cb;
+    cb.reset();
+    for (int i = objBegin; i < objEnd; ++i) cb.extendPoint(obj_centroids[ids[i]].v);
+    const float inv_parent_area = (float)(1.0f / nodes[nodeID].bounds.area());
+    float best_cost = n * kCIsec;
+    int best_axis = -1;
+    float best_pos = FLT_MAX;
+    const int S = 16;  // BVH_num_samples (DynBVH.cc:34)
+    float width[3], scale[3];
+    for (int a = 0; a < 3; ++a) {
+      width[a] = cb.hi[a] - cb.lo[a];
+      scale[a] = ((float)S / width[a]) * .999f;
+    }
+    const int axis = longestAxis(cb);
+    Box bins[S];
+    int counts[S];
+    for (int b = 0; b < S; ++b) {
+      bins[b].reset();
+      counts[b] = 0;
+    }
+    for (int i = objBegin; i < objEnd; ++i) {
+      const Box& ob = obj_bounds[ids[i]];
+      const float c = obj_centroids[ids[i]].v[axis];
+      // int(NaN) is INT_MIN on x86 (cvttss2si), clamped to bin 0 like the device's 0.
+      const float f = (c - cb.lo[axis]) * scale[axis];
+      int which = (f != f) ? 0 : (int)f;
+      which = which < 0 ? 0 : (which > S - 1 ? S - 1 : which);
+      counts[which]++;
+      bins[which].extendBox(ob);
+    }
+    float left_areas[S], right_areas[S];
+    int left_counts[S], right_counts[S];
+    Box left_box, right_box;
+    left_box.reset();
+    right_box.reset();
+    int num_left = 0, num_right = n;
+    for (int i = 0; i < S; ++i) {
+      num_left += counts[i];
+      num_right -= counts[i];
+      left_box.extendBox(bins[i]);
+      left_counts[i] = num_left;
+      right_counts[i] = num_right;
+      left_areas[i] = (float)left_box.area();
+    }
+    for (int i = S - 1; i >= 0; --i) {
+      right_areas[i] = (i == S - 1) ? FLT_MAX : (float)right_box.area();
+      right_box.extendBox(bins[i]);
+    }
+    for (int i = 0; i < S - 1; ++i) {
+      const float cost = (left_areas[i] * left_counts[i] + right_areas[i] * right_counts[i]) *
+                             inv_parent_area * kCIsec + kCTrav;
+      if (cost < best_cost) {
+        best_cost = cost;
+        best_axis = axis;
+        best_pos = cb.lo[axis] + width[axis] * (static_cast<float>(i + 1) / S);
+      }
+    }
+    p.axis = best_axis;
+    if (p.axis != -1) {
+      int middle = partitionObjects(objBegin, objEnd, p.axis, best_pos);
+      if (middle == objBegin || middle == objEnd) {
+        if (n < 3) {
+          p.axis = -1;
+          p.split = 0;
+          return p;
+        }
+        middle = objBegin + n / 2;
+      }
+      for (int i = objBegin; i < middle; ++i) p.bounds_l.extendBox(obj_bounds[ids[i]]);
+      for (int i = middle; i < objEnd; ++i) p.bounds_r.extendBox(obj_bounds[ids[i]]);
+      p.split = middle;
+      return p;
+    }
+    p.split = 0;
+    return p;
+  }
+
+  // DynBVH::build with useApproximateBuild (DynBVH.cc:1801-1857, TREE_ROT parts)
+  void buildApprox(int nodeID, int objBegin, int objEnd, bool exists) {
+    Partition p = partitionApprox(nodeID, objBegin, objEnd);
+    Node32& node = nodes[nodeID];
+    if (p.axis == -1) {
+      if (exists) return;
+      node.child = objBegin;
+      node.children = (int16_t)(objEnd - objBegin);
+      node.axis = 0;
+      costs[nodeID] = node.children * kCIsec;
+      ssize[nodeID] = 1;
+      std::sort(ids + objBegin, ids + objEnd);
+    } else {
+      const int my_spot = next_free;
+      next_free += 2;
+      node.child = my_spot;
+      node.children = 0;
+      node.axis = (uint8_t)p.axis;
+      nodes[my_spot + 0].bounds = p.bounds_l;
+      nodes[my_spot + 1].bounds = p.bounds_r;
+      buildApprox(my_spot + 0, objBegin, p.split, false);
+      buildApprox(my_spot + 1, p.split, objEnd, false);
+      ssize[nodeID] = 1 + ssize[my_spot + 0] + ssize[my_spot + 1];
+    }
+  }
+
+  void swapRec(int a, int b) {
+    std::swap(nodes[a], nodes[b]);
+    std::swap(costs[a], costs[b]);
+    std::swap(ssize[a], ssize[b]);
+  }
+
+  // DynBVH.cc:3051-3372
+  void rotateNode(int nodeID) {
+    const int leftID = nodes[nodeID].child;
+    const int rightID = nodes[nodeID].child + 1;
+    if (nodes[leftID].isLeaf() && nodes[leftID].children > 1)
+      buildApprox(leftID, nodes[leftID].child, nodes[leftID].child + nodes[leftID].children, true);
+    if (nodes[rightID].isLeaf() && nodes[rightID].children > 1)
+      buildApprox(rightID, nodes[rightID].child, nodes[rightID].child + nodes[rightID].children,
+                  true);
+    const int LL = nodes[leftID].child, LR = nodes[leftID].child + 1;
+    const int RL = nodes[rightID].child, RR = nodes[rightID].child + 1;
+    const float nodeArea = (float)nodes[nodeID].bounds.area();
+    const float leftArea = (float)nodes[leftID].bounds.area();
+    const float rightArea = (float)nodes[rightID].bounds.area();
+    float llA = 0.f, lrA = 0.f, rlA = 0.f, rrA = 0.f;
+    Box boxA, boxB;
+    boxA.reset();
+    boxB.reset();
+    float areaA = 0.f, areaB = 0.f;
+    int best = 0;
+    float bestCost = nodeArea * kCTrav + leftArea * costs[leftID] + rightArea * costs[rightID];
+    auto unite = [&](int a, int b) {
+      Box x = nodes[a].bounds;
+      x.extendBox(nodes[b].bounds);
+      return x;
+    };
+    if (!nodes[leftID].isLeaf()) {
+      llA = (float)nodes[LL].bounds.area();
+      lrA = (float)nodes[LR].bounds.area();
+      const float partial = nodeArea * kCTrav + rightArea * costs[rightID] + lrA * costs[LR] +
+                            llA * costs[LL];
+      const Box b1 = unite(rightID, LR);
+      const float a1 = (float)b1.area();
+      const float c1 = a1 * kCTrav + partial;
+      if (c1 < bestCost) { best = 1; bestCost = c1; boxA = b1; areaA = a1; }
+      const Box b2 = unite(rightID, LL);
+      const float a2 = (float)b2.area();
+      const float c2 = a2 * kCTrav + partial;
+      if (c2 < bestCost) { best = 2; bestCost = c2; boxA = b2; areaA = a2; }
+    }
+    if (!nodes[rightID].isLeaf()) {
+      rlA = (float)nodes[RL].bounds.area();
+      rrA = (float)nodes[RR].bounds.area();
+      const float partial = nodeArea * kCTrav + leftArea * costs[leftID] + rrA * costs[RR] +
+                            rlA * costs[RL];
+      const Box b3 = unite(leftID, RR);
+      const float a3 = (float)b3.area();
+      const float c3 = a3 * kCTrav + partial;
+      if (c3 < bestCost) { best = 3; bestCost = c3; boxA = b3; areaA = a3; }
+      const Box b4 = unite(leftID, RL);
+      const float a4 = (float)b4.area();
+      const float c4 = a4 * kCTrav + partial;
+      if (c4 < bestCost) { best = 4; bestCost = c4; boxA = b4; areaA = a4; }
+      if (!nodes[leftID].isLeaf()) {
+        const float partial2 = nodeArea * kCTrav + llA * costs[LL] + lrA * costs[LR] +
+                               rlA * costs[RL] + rrA * costs[RR];
+        const Box b5l = unite(LL, RL), b5r = unite(LR, RR);
+        const float a5l = (float)b5l.area(), a5r = (float)b5r.area();
+        const float c5 = (a5l + a5r) * kCTrav + partial2;
+        if (c5 < bestCost) { best = 5; bestCost = c5; boxA = b5l; areaA = a5l; boxB = b5r; areaB = a5r; }
+        const Box b6l = unite(LL, RR), b6r = unite(RL, LR);
+        const float a6l = (float)b6l.area(), a6r = (float)b6r.area();
+        const float c6 = (a6l + a6r) * kCTrav + partial2;
+        if (c6 < bestCost) { best = 6; bestCost = c6; boxA = b6l; areaA = a6l; boxB = b6r; areaB = a6r; }
+      }
+    }
+    costs[nodeID] = bestCost / nodeArea;
+    if (best == 0) return;
+    const float invA = 1 / areaA;
+    switch (best) {
+      case 1:
+        nodes[leftID].bounds = boxA;
+        costs[leftID] = kCTrav + ((lrA * costs[LR] + rightArea * costs[rightID]) * invA);
+        swapRec(LL, rightID);
+        ssize[leftID] = ssize[LL] + ssize[LR] + 1;
+        break;
+      case 2:
+        nodes[leftID].bounds = boxA;
+        costs[leftID] = kCTrav + ((llA * costs[LL] + rightArea * costs[rightID]) * invA);
+        swapRec(LR, rightID);
+        ssize[leftID] = ssize[LL] + ssize[LR] + 1;
+        break;
+      case 3:
+        nodes[rightID].bounds = boxA;
+        costs[rightID] = kCTrav + ((rrA * costs[RR] + leftArea * costs[leftID]) * invA);
+        swapRec(RL, leftID);
+        ssize[rightID] = ssize[RL] + ssize[RR] + 1;
+        break;
+      case 4:
+        nodes[rightID].bounds = boxA;
+        costs[rightID] = kCTrav + ((rlA * costs[RL] + leftArea * costs[leftID]) * invA);
+        swapRec(RR, leftID);
+        ssize[rightID] = ssize[RL] + ssize[RR] + 1;
+        break;
+      case 5:
+        nodes[leftID].bounds = boxA;
+        nodes[rightID].bounds = boxB;
+        costs[leftID] = kCTrav + ((llA * costs[LL] + rlA * costs[RL]) * invA);
+        costs[rightID] = kCTrav + ((lrA * costs[LR] + rrA * costs[RR]) / areaB);
+        swapRec(LR, RL);
+        ssize[leftID] = ssize[LL] + ssize[LR] + 1;
+        ssize[rightID] = ssize[RL] + ssize[RR] + 1;
+        break;
+      default:  // 6
+        nodes[leftID].bounds = boxA;
+        nodes[rightID].bounds = boxB;
+        costs[leftID] = kCTrav + ((llA * costs[LL] + rrA * costs[RR]) * invA);
+        costs[rightID] = kCTrav + ((lrA * costs[LR] + rlA * costs[RL]) / areaB);
+        swapRec(LR, RR);
+        ssize[leftID] = ssize[LL] + ssize[LR] + 1;
+        ssize[rightID] = ssize[RL] + ssize[RR] + 1;
+        break;
+    }
+    // Step 3: guess split axes (overlap of the two children, DynBVH.cc:3305-3352)
+    auto guess = [&](int parent, int a, int b) {
+      const Box& x = nodes[a].bounds;
+      const Box& y = nodes[b].bounds;
+      float o[3];
+      for (int k = 0; k < 3; ++k) o[k] = fmax_(x.lo[k], y.lo[k]) - fmin_(x.hi[k], y.hi[k]);
+      const int ax = indexOfMax(o[0], o[1], o[2]);
+      nodes[parent].axis = (uint8_t)ax;
+      if (nodes[a].bounds.hi[ax] > nodes[b].bounds.hi[ax]) swapRec(a, b);
+    };
+    if (best <= 2 || best >= 5) guess(leftID, LL, LR);
+    if (best >= 3) guess(rightID, RL, RR);
+    guess(nodeID, leftID, rightID);
+    if (nodes[nodeID].bits & 2) {
+      auto mark = [&](int r) {
+        if (ssize[r] > large_subtree)
+          nodes[r].bits |= 2;
+        else
+          nodes[r].bits &= ~2;
+      };
+      mark(leftID);
+      mark(rightID);
+      if (best <= 2 || best >= 5) {
+        mark(LL);
+        mark(LR);
+      }
+      if (best >= 3) {
+        mark(RL);
+        mark(RR);
+      }
+    }
+  }
+
+  // DynBVH.cc:2078-2123, updateBounds<false> with TREE_ROT
+  void updateBounds(int id) {
+    Node32& node = nodes[id];
+    if (node.isLeaf()) {
+      node.bounds.reset();
+      for (int i = 0; i < node.children; ++i) node.bounds.extendBox(obj_bounds[ids[node.child + i]]);
+      node.axis = 0;
+    } else {
+      const int l = node.child, r = node.child + 1;
+      updateBounds(l);
+      updateBounds(r);
+      node.bounds.reset();
+      node.bounds.extendBox(nodes[l].bounds);
+      node.bounds.extendBox(nodes[r].bounds);
+      rotateNode(id);
+    }
+  }
+
+  // DynBVH.cc:2018-2030 then the numProcs == 1 branch
+  void update() {
+    const int N = mesh.ntris;
+    obj_bounds.resize(N);
+    obj_centroids.resize(N);
+    for (int i = 0; i < N; ++i) {
+      obj_bounds[i] = triBox(mesh, i);
+      for (int a = 0; a < 3; ++a)
+        obj_centroids[i].v[a] = obj_bounds[i].hi[a] * 0.5f + obj_bounds[i].lo[a] * (1 - 0.5f);
+    }
+    updateBounds(0);
+  }
+};
+
+// DynBVH::computeCost<true> (DynBVH.cc:3388-3415) and subtree sizes (DynBVH.cc:1971-1996).
+float rotCost(const Node32* nodes, float* costs, int id) {
+  const Node32& node = nodes[id];
+  float cost;
+  if (node.isLeaf()) {
+    cost = node.children * kCIsec;
+  } else {
+    const float lc = rotCost(nodes, costs, node.child);
+    const float rc = rotCost(nodes, costs, node.child + 1);
+    const float area = (float)node.bounds.area();
+    const float la = (float)nodes[node.child].bounds.area();
+    const float ra = (float)nodes[node.child + 1].bounds.area();
+    cost = kCTrav + (la * lc + ra * rc) / area;
+  }
+  costs[id] = cost;
+  return cost;
+}
+uint32_t rotSizes(const Node32* nodes, uint32_t* ssize, int id) {
+  const Node32& node = nodes[id];
+  uint32_t s = 1;
+  if (!node.isLeaf()) s += rotSizes(nodes, ssize, node.child) + rotSizes(nodes, ssize, node.child + 1);
+  ssize[id] = s;
+  return s;
+}
+
 extern "C" {
 
 int oracle_node_size(void) { return (int)sizeof(Node32); }
@@ -1113,6 +1517,27 @@ void oracle_refit(const float* verts, int nverts, const uint32_t* idx, int ntris
     }
   } rr{nodes, ids, ob};
   rr.go(0);
+}
+
+// Rotation state after DynBVH::rebuild with TREE_ROT: costs (computeCost<true>(0)) and
+// subtree sizes of the reachable records; the rest stay zero.
+void oracle_rot_init(const void* nodes_v, int n_records, float* costs, uint32_t* ssize) {
+  const Node32* nodes = (const Node32*)nodes_v;
+  for (int i = 0; i < n_records; ++i) {
+    costs[i] = 0.f;
+    ssize[i] = 0;
+  }
+  rotCost(nodes, costs, 0);
+  rotSizes(nodes, ssize, 0);
+}
+
+// One DynBVH::update (TREE_ROT, numProcs == 1) on the current vertices, in place: nodes
+// (2T records), object ids, costs, subtree sizes and the next free record pair.
+void oracle_update(const float* verts, int nverts, const uint32_t* idx, int ntris, void* nodes_v,
+                   int32_t* ids, float* costs, uint32_t* ssize, int* next_free) {
+  Mesh m{verts, nverts, idx, ntris};
+  Updater u(m, (Node32*)nodes_v, ids, costs, ssize, *next_free, (uint32_t)(2 * (size_t)ntris / 20));
+  u.update();
 }
 
 // Wald per-triangle constants (11 floats + k as float) for tests of the setup pass.

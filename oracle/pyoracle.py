@@ -46,6 +46,10 @@ def load() -> C.CDLL:
             C.c_int64, C.c_int, C.c_int, P, P, P, P, P]
         lib.oracle_refit.restype = None
         lib.oracle_refit.argtypes = [P, C.c_int, P, C.c_int, P, P]
+        lib.oracle_rot_init.restype = None
+        lib.oracle_rot_init.argtypes = [P, C.c_int, P, P]
+        lib.oracle_update.restype = None
+        lib.oracle_update.argtypes = [P, C.c_int, P, C.c_int, P, P, P, P, P]
         lib.oracle_wald_setup.restype = None
         lib.oracle_wald_setup.argtypes = [P, P, C.c_int, P]
         lib.oracle_node_size.restype = C.c_int
@@ -135,6 +139,26 @@ def refit(verts, idx, nodes, ids):
     assert nodes.flags["C_CONTIGUOUS"]
     load().oracle_refit(_p(v), len(v), _p(i), len(i), _p(nodes),
                         _p(np.ascontiguousarray(ids, np.int32)))
+
+
+class RotState:
+    """DynBVH state for updates with tree rotations: records (2T), object ids, costs,
+    subtree sizes and the next free record pair (DynBVH.h:377-385)."""
+
+    def __init__(self, nodes, ids, num_nodes):
+        self.nodes = np.ascontiguousarray(nodes).copy()
+        self.ids = np.ascontiguousarray(ids, np.int32).copy()
+        n = len(self.nodes)
+        self.costs = np.zeros(n, np.float32)
+        self.ssize = np.zeros(n, np.uint32)
+        self.next_free = C.c_int(int(num_nodes))
+        load().oracle_rot_init(_p(self.nodes), n, _p(self.costs), _p(self.ssize))
+
+    def update(self, verts, idx):
+        """One DynBVH::update with rotations and leaf splits, in place."""
+        v, i = _mesh(verts, idx)
+        load().oracle_update(_p(v), len(v), _p(i), len(i), _p(self.nodes), _p(self.ids),
+                             _p(self.costs), _p(self.ssize), C.byref(self.next_free))
 
 
 def ttp_probs(parent, left, right):

@@ -47,7 +47,7 @@ class Hits(C.Structure):
 
 class TraceOpts(C.Structure):
     _fields_ = [("packet_size", C.c_int), ("stack_lds", C.c_int), ("blocks_per_cu", C.c_int),
-                ("counters", C.c_void_p)]
+                ("counters", C.c_void_p), ("layout", C.c_int)]
 
 
 class Batch(C.Structure):
@@ -70,6 +70,7 @@ class LaunchConfig(C.Structure):
 
 MH_LAYOUT_PAIR = 0
 MH_LAYOUT_REF = 1
+MH_LAYOUT_QUAD = 2
 
 
 class RayBuffers(C.Structure):

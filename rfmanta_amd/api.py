@@ -131,12 +131,13 @@ class DeviceHits:
 
 
 def trace_opts(packet_size: int = 1, stack_lds: int = 0, blocks_per_cu: int = 0,
-               counters: DeviceArray | None = None) -> L.TraceOpts:
+               counters: DeviceArray | None = None, layout: int = 0) -> L.TraceOpts:
     o = L.TraceOpts()
     o.packet_size = packet_size
     o.stack_lds = stack_lds
     o.blocks_per_cu = blocks_per_cu
     o.counters = counters.ptr if counters is not None else None
+    o.layout = layout
     return o
 
 
@@ -243,7 +244,7 @@ class Scene:
                                   C.byref(opts) if opts is not None else None, stream))
 
     def trace_host(self, rays: dict, any_hit: bool = False, packet_size: int = 1,
-                   stack_lds: int = 0) -> dict:
+                   stack_lds: int = 0, layout: int = 0) -> dict:
         n = len(rays["ox"])
         arr = {k: np.ascontiguousarray(rays[k], np.float32) for k in RAY_FIELDS}
         m = rays.get("masked")
@@ -255,7 +256,7 @@ class Scene:
         r = L.Rays(*[arr[k].ctypes.data for k in RAY_FIELDS], L.ptr(m))
         h = L.Hits(out["prim"].ctypes.data, out["t"].ctypes.data, out["u"].ctypes.data,
                    out["v"].ctypes.data, out["occluded"].ctypes.data if any_hit else None)
-        o = trace_opts(packet_size, stack_lds)
+        o = trace_opts(packet_size, stack_lds, layout=layout)
         L.check(L.load().mh_trace_host(self.h, C.byref(r), C.byref(h), n,
                                        L.MH_ANY_HIT if any_hit else L.MH_CLOSEST_HIT, C.byref(o)))
         return out

@@ -17,12 +17,11 @@
 // The last wave to leave resets the counters, so consecutive launches (and graph
 // replays) need no clearing pass.
 //
-// Two node layouts, both giving the reference's one-ray results and visit order
-// (DynBVH::intersectNode on a one-ray packet, Model/Groups/DynBVH.cc:274-404):
-//   PAIR: 64-byte PairNodes hold both child boxes; the far child is pushed with its entry
-//         distance and skipped at pop time if that exceeds the current minT.
-//   REF:  the reference's 32-byte records; each node's own box is tested when it is
-//         visited, exactly as firstIntersects does.
+// Three node layouts, all giving the reference's one-ray results and visit order
+// (DynBVH::intersectNode on a one-ray packet, Model/Groups/DynBVH.cc:274-404; the
+// traversal loops are traverse.h's): PAIR (64-byte records with both child boxes), REF
+// (the reference's 32-byte records) and QUAD (128-byte records with four grandchild
+// boxes, whole-wave schedules only).
 // Triangle tests and arithmetic are those of trace.hip (shared header).
 //
 // A table whose batches are all closest-hit (or all any-hit) runs a kernel specialised to
@@ -101,7 +100,7 @@ __device__ __forceinline__ bool step(const DevScene& sc, Lane& L, LaneStack<S, B
       const bool h0 = slab(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
       const bool h1 = slab(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
       const bool one_first =
-          L.any ? ((M.z >> 2) & 1) == 0 : !(pick3(M.z & 3, L.r.d[0], L.r.d[1], L.r.d[2]) > 0.f);
+          L.any ? ((M.z >> 2) & 1) == 0 : ((L.r.dpos >> (M.z & 3)) & 1) == 0;
       const int fref = one_first ? M.y : M.x;
       const int bref = one_first ? M.x : M.y;
       const bool fh = one_first ? h1 : h0;
@@ -145,7 +144,7 @@ __device__ __forceinline__ bool step(const DevScene& sc, Lane& L, LaneStack<S, B
         if (done) return false;
       } else {
         const int front = L.any ? ((nd.bits & 1) ? 0 : 1)
-                                : (pick3(nd.axis, L.r.d[0], L.r.d[1], L.r.d[2]) > 0.f ? 0 : 1);
+                                : (((L.r.dpos >> (nd.axis & 3)) & 1) ? 0 : 1);
         st.push(nd.child + 1 - front, 0.f);
         L.cur = nd.child + front;
         return true;
@@ -223,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(DevScene sc,
   LaneStack<S, BLOCK> st(s_ring, reinterpret_cast<unsigned long long*>(spill), spill_stride,
                          gtid, tid);
 
-  if (MODE != kModeLane) {
+  if constexpr (MODE != kModeLane) {
     // A wave traces C consecutive ray numbers, 64 at a time, so its rays stay as coherent
     // as their batch. Its first chunk is wave_id * C; later chunks come from the counter
     // (dynamic) or the next stride (static). Each span of one batch inside a chunk runs
@@ -350,7 +349,12 @@ int launch_k(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
 // refill 64: whole-wave chunks (chunk 0: static stride); otherwise per-lane refill.
 template <int PRIM, int LAYOUT, int BLOCK>
 int launch_s(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
-  if (bl.refill < 64) return launch_w<PRIM, LAYOUT, BLOCK, kModeLane, kKindMixed>(sc, bl, s);
+  if (bl.refill < 64) {
+    // The per-lane state machine steps PAIR or REF nodes; QUAD runs whole-wave schedules.
+    if (LAYOUT == kLayoutQuad) return (int)hipErrorInvalidValue;
+    return launch_w<PRIM, LAYOUT == kLayoutQuad ? kLayoutPair : LAYOUT, BLOCK, kModeLane,
+                    kKindMixed>(sc, bl, s);
+  }
   return bl.chunk == 0 ? launch_k<PRIM, LAYOUT, BLOCK, kModeWaveStatic>(sc, bl, s)
                        : launch_k<PRIM, LAYOUT, BLOCK, kModeWave>(sc, bl, s);
 }
@@ -374,5 +378,7 @@ int launch_batch_ks_pair(const DevScene& sc, const BatchLaunch& bl, hipStream_t 
 int launch_batch_ks_ref(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 int launch_batch_wald_pair(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 int launch_batch_wald_ref(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
+int launch_batch_ks_quad(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
+int launch_batch_wald_quad(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 
 }  // namespace mh

@@ -149,6 +149,67 @@ int upload(mh_scene* s) {
     P.node = order[pi];
   }
   s->root_ref = encode(0);
+  // QuadNodes: one per inner record that is the root or a grandchild slot of a quad
+  // (two binary levels per record), in depth-first order of the quads.
+  std::vector<mh::QuadNode> quads;
+  std::vector<int32_t> quad_of(nrec, -1), qslot(nrec, -1);
+  uint32_t max_wdepth = 0;
+  if (!nodes[0].leaf()) {
+    quad_of[0] = 0;
+    quads.emplace_back();
+    std::vector<It> qs;
+    qs.push_back({0, 0});
+    while (!qs.empty()) {
+      const It it = qs.back();
+      qs.pop_back();
+      max_wdepth = std::max(max_wdepth, it.depth);
+      const int32_t q = quad_of[it.rec];
+      const mh::RefNode& n = nodes[it.rec];
+      mh::QuadNode Q{};
+      Q.node = it.rec;
+      Q.meta = (n.axis & 3) | ((n.bits & 1) << 2);
+      int32_t grand[4] = {-1, -1, -1, -1};
+      for (int side = 0; side < 2; ++side) {
+        const int32_t c = n.child + side;
+        const mh::RefNode& cn = nodes[c];
+        if (cn.leaf()) {
+          grand[2 * side] = c;
+        } else {
+          grand[2 * side] = cn.child;
+          grand[2 * side + 1] = cn.child + 1;
+          Q.meta |= ((cn.axis & 3) | ((cn.bits & 1) << 2)) << (3 + 3 * side);
+        }
+      }
+      for (int k = 0; k < 4; ++k) {
+        const int32_t g = grand[k];
+        if (g < 0) continue;
+        Q.meta |= 1 << (9 + k);
+        const mh::RefNode& gn = nodes[g];
+        float* b = Q.b + 6 * k;
+        b[0] = gn.box.lo[0];
+        b[1] = gn.box.hi[0];
+        b[2] = gn.box.lo[1];
+        b[3] = gn.box.hi[1];
+        b[4] = gn.box.lo[2];
+        b[5] = gn.box.hi[2];
+        qslot[g] = q * 4 + k;
+        if (gn.leaf()) {
+          Q.ref[k] = encode(g);
+        } else {
+          quad_of[g] = (int32_t)quads.size();
+          quads.emplace_back();
+          Q.ref[k] = quad_of[g];
+        }
+      }
+      quads[q] = Q;
+      for (int k = 3; k >= 0; --k)
+        if (grand[k] >= 0 && !nodes[grand[k]].leaf()) qs.push_back({grand[k], it.depth + 1});
+    }
+    s->quad_root = 0;
+  } else {
+    s->quad_root = encode(0);
+  }
+  s->quad_stack = 3 * (max_wdepth + 1);
   s->n_inner = (uint32_t)order.size();
   s->n_leaves = (uint32_t)leaves.size();
   s->depth = max_depth;
@@ -166,6 +227,8 @@ int upload(mh_scene* s) {
   if ((rc = s->ids.reserve((size_t)T * sizeof(int32_t)))) return rc;
   if ((rc = s->big.reserve((size_t)T * sizeof(int32_t)))) return rc;
   if ((rc = s->slot.reserve(nrec * sizeof(int32_t)))) return rc;
+  if ((rc = s->qslot.reserve(nrec * sizeof(int32_t)))) return rc;
+  if ((rc = s->quads.reserve(std::max<size_t>(1, quads.size()) * sizeof(mh::QuadNode)))) return rc;
   if ((rc = s->leaves.reserve(std::max<size_t>(1, leaves.size()) * sizeof(int32_t)))) return rc;
   if ((rc = s->levels.reserve(std::max<size_t>(1, levels.size()) * sizeof(int32_t)))) return rc;
   HIP_OK(hipMemcpy(s->refs.p, nodes.data(), nrec * sizeof(mh::RefNode), hipMemcpyHostToDevice));
@@ -175,6 +238,10 @@ int upload(mh_scene* s) {
   HIP_OK(hipMemcpy(s->ids.p, s->bvh.ids.data(), (size_t)T * 4, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->big.p, big.data(), (size_t)T * 4, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->slot.p, slot.data(), nrec * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->qslot.p, qslot.data(), nrec * 4, hipMemcpyHostToDevice));
+  if (!quads.empty())
+    HIP_OK(hipMemcpy(s->quads.p, quads.data(), quads.size() * sizeof(mh::QuadNode),
+                     hipMemcpyHostToDevice));
   if (!leaves.empty())
     HIP_OK(hipMemcpy(s->leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice));
   if (!levels.empty())
@@ -233,6 +300,10 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
   mh::LaunchCfg cfg{};
   cfg.any_hit = any;
   cfg.packet_size = psize;
+  cfg.layout = opts ? opts->layout : MH_LAYOUT_PAIR;
+  if (cfg.layout != MH_LAYOUT_PAIR && cfg.layout != MH_LAYOUT_QUAD)
+    return fail(MH_E_INVALID, "trace layout %d is not MH_LAYOUT_PAIR or MH_LAYOUT_QUAD",
+                cfg.layout);
   cfg.counters = opts ? opts->counters : nullptr;
   if (psize > 1) {
     if (s->depth + 2 > 128)
@@ -245,7 +316,8 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
     cfg.blocks = (int)std::max<int64_t>(1, std::min(want, cap));
     cfg.stack_lds = S;
     cfg.spill_stride = (int64_t)cfg.blocks * 256;
-    const int64_t need = (int64_t)s->depth > S ? (int64_t)s->depth - S : 0;
+    const int layout = cfg.counters ? MH_LAYOUT_PAIR : cfg.layout;
+    const int64_t need = std::max<int64_t>(0, (int64_t)s->stack_bound(layout) - S);
     if (need > 0) {
       std::lock_guard<std::mutex> lk(s->spill_mu);
       DevBuf*& b = s->spill[stream];
@@ -521,7 +593,9 @@ int mh_scene_refit(mh_scene* s, void* stream) {
   r.obj_ids = s->ids.as<int32_t>();
   r.verts = s->d_verts.as<float>();
   r.idx = s->d_idx.as<uint32_t>();
+  r.quads = s->quads.as<mh::QuadNode>();
   r.slot = s->slot.as<int32_t>();
+  r.qslot = s->qslot.as<int32_t>();
   r.leaves = s->leaves.as<int32_t>();
   r.levels = s->levels.as<int32_t>();
   r.n_leaves = (int32_t)s->n_leaves;

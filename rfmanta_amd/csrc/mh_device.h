@@ -4,13 +4,16 @@
 //   refs      RefNode[2T]      reference DynBVH records, 32 B (packet traversal, refit)
 //   pairs     PairNode[I]      one 64-B record per inner node holding both child boxes
 //                              (single-ray traversal; one 64-B fetch decides both children)
+//   quads     QuadNode[Q]      one 128-B record per inner node at an even depth below the
+//                              root or a quad child: the boxes of its grandchildren (or of
+//                              a leaf child), so one fetch replaces two binary levels
 //   tris      float4[3T]       per object slot (object_ids order), 48 B:
 //                              Kensler-Shirley: p0, e0 = p1-p0, e1 = p0-p2, n = e0 x e1
 //                              Wald: n_u n_v n_d k | b_nu b_nv b_d n_k | c_nu c_nv c_d 0
 //   obj_ids   int32[T]         object_ids
 //   big_count int32[T]         leaf size at a leaf's first slot when it is >= 15
 //   verts     float[3V], idx uint32[3T], base_verts float[3V]    mesh (refit, animation)
-//   parent    int32[2T], slot int32[2T], leaves int32[L], arrive uint32[2T]   refit links
+//   slot, qslot int32[2T], leaves int32[L], levels int32[I]           refit links
 #pragma once
 
 #include <cstdint>
@@ -34,13 +37,29 @@ struct alignas(16) PairNode {
 };
 static_assert(sizeof(PairNode) == 64, "pair node is 64 bytes");
 
+// Two binary levels in one record. For an inner record N with children A = child and
+// B = child + 1, slots 0-1 hold A's children (or A itself in slot 0 when A is a leaf) and
+// slots 2-3 hold B's children (or B). A slot's reference is a quad index (>= 0) or a leaf
+// code as in PairNode. meta: bits 0-1 N's split axis, bit 2 N's isLeftCheaper, bits 3-4
+// and 5 the same for A, bits 6-7 and 8 for B, bits 9-12 slot in use.
+struct alignas(16) QuadNode {
+  float b[24];  // per slot: lo.x hi.x lo.y hi.y lo.z hi.z
+  int32_t ref[4];
+  int32_t meta;
+  int32_t node;  // reference record id of N
+  int32_t pad[2];
+};
+static_assert(sizeof(QuadNode) == 128, "quad node is 128 bytes");
+
 struct DevScene {
   const PairNode* pairs;
+  const QuadNode* quads;
   const RefNode* refs;
   const float* tris;  // 12 floats per slot
   const int32_t* obj_ids;
   const int32_t* big_count;
   int32_t root_ref;   // pair-encoded reference of record 0
+  int32_t quad_root;  // quad-encoded reference of record 0
   int32_t prim_kind;  // 0 KS, 1 Wald
 };
 
@@ -57,6 +76,7 @@ struct DevHits {
 
 struct LaunchCfg {
   int any_hit;
+  int layout;         // kLayoutPair or kLayoutQuad (single-ray)
   int packet_size;    // 1 = single-ray kernel
   int stack_lds;      // 8/16/32 (single-ray)
   int blocks;         // grid size in blocks of 256
@@ -65,9 +85,10 @@ struct LaunchCfg {
   int64_t spill_stride;
 };
 
-// Node layouts of the persistent batch kernel (batch.hip).
+// Node layouts of the single-ray traversal (MH_LAYOUT_*).
 constexpr int kLayoutPair = 0;  // 64-byte PairNodes, both child boxes per fetch
 constexpr int kLayoutRef = 1;   // 32-byte reference records, own box tested at visit
+constexpr int kLayoutQuad = 2;  // 128-byte QuadNodes, four grandchild boxes per fetch
 
 // One ray batch of a persistent launch: rays [start, start + count) of the launch's
 // global numbering are rays [0, count) of this batch.
@@ -118,7 +139,9 @@ struct DevRefit {
   const int32_t* obj_ids;
   const float* verts;
   const uint32_t* idx;
+  QuadNode* quads;
   const int32_t* slot;    // pair index * 2 + side for a non-root record, -1 for the root
+  const int32_t* qslot;   // quad index * 4 + slot for a record held by a quad, else -1
   const int32_t* leaves;  // leaf record ids
   const int32_t* levels;  // inner record ids grouped by depth (level_off gives the groups)
   int32_t n_leaves;

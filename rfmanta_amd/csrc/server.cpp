@@ -509,8 +509,8 @@ int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_serve
     HIP_OK(hipHostMalloc(&p.host_table, table_bytes(), hipHostMallocDefault));
     p.blocks = s->launch.blocks_per_cu * mh_internal::cu_count();
     p.spill_stride = (int64_t)p.blocks * s->launch.block;
-    const int64_t need = (int64_t)scenes[k]->depth > s->launch.stack_lds
-                             ? (int64_t)scenes[k]->depth - s->launch.stack_lds : 0;
+    const int64_t need = std::max<int64_t>(
+        0, (int64_t)scenes[k]->stack_bound(s->launch.layout) - s->launch.stack_lds);
     if (need > 0 && (rc = p.spill.reserve((size_t)need * p.spill_stride * 8))) return rc;
   }
   mh_server* raw = s.get();

@@ -1,8 +1,8 @@
 // Traversal kernels for gfx950 (wave64).
 //
 // k_trace_ray: one ray per lane, DynBVH::intersect on a one-ray packet
-//   (Model/Groups/DynBVH.cc:101-404) over 64-byte PairNodes with an LDS ring stack; the
-//   traversal itself is traverse.h's, shared with the batch kernel.
+//   (Model/Groups/DynBVH.cc:101-404) over 64-byte PairNodes or 128-byte QuadNodes with an
+//   LDS ring stack; the traversal itself is traverse.h's, shared with the batch kernel.
 //
 // k_trace_packet: one RayPacket of up to 64 consecutive rays per wave, the reference's
 //   packet recursion (intersectNode/firstIntersects/lastIntersects, DynBVH.cc:274-905,
@@ -33,7 +33,7 @@ constexpr int kPacketStack = 128;
 // ---------------------------------------------------------------------------
 // Single-ray traversal (traverse.h), grid-stride over the rays
 // ---------------------------------------------------------------------------
-template <bool ANY, int PRIM, int S, bool STATS>
+template <bool ANY, int PRIM, int LAYOUT, int S, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_trace_ray(DevScene sc, DevRays rays, DevHits hits,
                                                       int64_t n, int2* spill,
                                                       int64_t spill_stride,
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_ray(DevScene sc, DevRays rays,
   const GRays gr = global_view(rays);
   const GHits gh = global_view(hits);
   for (int64_t ri = gtid; ri < n; ri += stride)
-    trace_store<ANY, PRIM, kLayoutPair, S, kBlock, STATS>(sc, root, gr, gh, ri, st, nbox, ntri);
+    trace_store<ANY, PRIM, LAYOUT, S, kBlock, STATS>(sc, root, gr, gh, ri, st, nbox, ntri);
   if (STATS && counters) {
     atomicAdd(counters + 0, nbox);
     atomicAdd(counters + 1, ntri);
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_packet(DevScene sc, DevRays ra
   }
 }
 
-template <bool ANY, int PRIM, bool STATS>
+template <bool ANY, int PRIM, int LAYOUT, bool STATS>
 int launch_ray_s(const DevScene& sc, const DevRays& rays, const DevHits& hits, int64_t n,
                  const LaunchCfg& cfg, hipStream_t s) {
   unsigned long long* c = reinterpret_cast<unsigned long long*>(cfg.counters);
@@ -239,15 +239,15 @@ int launch_ray_s(const DevScene& sc, const DevRays& rays, const DevHits& hits, i
   dim3 g(cfg.blocks), b(kBlock);
   switch (cfg.stack_lds) {
     case 8:
-      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, 8, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
+      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, LAYOUT, 8, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
                          cfg.spill_stride, c);
       break;
     case 32:
-      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, 32, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
+      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, LAYOUT, 32, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
                          cfg.spill_stride, c);
       break;
     default:
-      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, 16, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
+      hipLaunchKernelGGL((k_trace_ray<ANY, PRIM, LAYOUT, 16, STATS>), g, b, 0, s, sc, rays, hits, n, sp,
                          cfg.spill_stride, c);
       break;
   }
@@ -270,8 +270,10 @@ int launch_any_prim(const DevScene& sc, const DevRays& rays, const DevHits& hits
                          dim3(kBlock), 0, s, sc, rays, hits, n, cfg.packet_size, c);
     return (int)hipGetLastError();
   }
-  return stats ? launch_ray_s<ANY, PRIM, true>(sc, rays, hits, n, cfg, s)
-               : launch_ray_s<ANY, PRIM, false>(sc, rays, hits, n, cfg, s);
+  // The statistics variant counts the binary traversal's tests (PAIR layout).
+  if (stats) return launch_ray_s<ANY, PRIM, kLayoutPair, true>(sc, rays, hits, n, cfg, s);
+  return cfg.layout == kLayoutQuad ? launch_ray_s<ANY, PRIM, kLayoutQuad, false>(sc, rays, hits, n, cfg, s)
+                                   : launch_ray_s<ANY, PRIM, kLayoutPair, false>(sc, rays, hits, n, cfg, s);
 }
 
 }  // namespace

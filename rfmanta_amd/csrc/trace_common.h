@@ -53,6 +53,7 @@ static __device__ __forceinline__ float pick3(int k, float a, float b, float c) 
 
 struct Ray {
   float o[3], d[3], inv[3];
+  unsigned dpos;  // bit k set iff d[k] > 0 (front child choice on split axis k)
 };
 
 // Slab test of DynBVH::firstIntersects (scalar loop, DynBVH.cc:666-680).
@@ -79,6 +80,36 @@ static __device__ __forceinline__ bool slab(float lx, float hx, float ly, float 
   tmax = (fr < tmax) ? fr : tmax;
   tnear = tmin;
   return tmin <= tmax;
+}
+
+// The same test with min/max instructions. When the origin and the reciprocal direction
+// are finite (and the box coordinates are), no t is NaN, and then each select of the
+// reference's form equals IEEE min/max: both return one of their inputs, and zero signs
+// cannot matter because tmin starts at T_EPSILON > 0. Results are identical bit for bit.
+static __device__ __forceinline__ bool slab_fast(float lx, float hx, float ly, float hy, float lz,
+                                          float hz, const Ray& r, float minT, float& tnear) {
+  const float ax = (lx - r.o[0]) * r.inv[0], bx = (hx - r.o[0]) * r.inv[0];
+  const float ay = (ly - r.o[1]) * r.inv[1], by = (hy - r.o[1]) * r.inv[1];
+  const float az = (lz - r.o[2]) * r.inv[2], bz = (hz - r.o[2]) * r.inv[2];
+  const float tmin = fmaxf(fmaxf(fmaxf(kEps, fminf(ax, bx)), fminf(ay, by)), fminf(az, bz));
+  const float tmax = fminf(fminf(fminf(minT, fmaxf(ax, bx)), fmaxf(ay, by)), fmaxf(az, bz));
+  tnear = tmin;
+  return tmin <= tmax;
+}
+
+template <bool FAST>
+static __device__ __forceinline__ bool slab_v(float lx, float hx, float ly, float hy, float lz,
+                                       float hz, const Ray& r, float minT, float& tnear) {
+  return FAST ? slab_fast(lx, hx, ly, hy, lz, hz, r, minT, tnear)
+              : slab(lx, hx, ly, hy, lz, hz, r, minT, tnear);
+}
+
+// A ray for which slab_fast is exact: finite origin and reciprocal direction, minT not NaN.
+static __device__ __forceinline__ bool regular_ray(const Ray& r, float minT) {
+  bool ok = !(minT != minT);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) ok = ok && fabsf(r.o[k]) <= 3.4028235e38f && fabsf(r.inv[k]) <= 3.4028235e38f;
+  return ok;
 }
 
 // KenslerShirleyTriangle::intersect, scalar branch (KenslerShirleyTriangle.cc:336-377),
@@ -159,6 +190,8 @@ static __device__ __forceinline__ void load_ray(const GRays& rays, int64_t i, Ra
   r.inv[0] = 1.0f / r.d[0];
   r.inv[1] = 1.0f / r.d[1];
   r.inv[2] = 1.0f / r.d[2];
+  r.dpos = (unsigned)(r.d[0] > 0.f) | ((unsigned)(r.d[1] > 0.f) << 1) |
+           ((unsigned)(r.d[2] > 0.f) << 2);
 }
 
 static __device__ __forceinline__ void store_hit(const GHits& h, int64_t i, int32_t prim, float t,

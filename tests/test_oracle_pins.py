@@ -16,6 +16,13 @@ oracle function is pinned by mathematics that an independent formulation fixes:
                             optimality of every split against a float64 SAH sweep
   RTSAH probabilities       Monte Carlo over isotropic random lines
   refit                     exact min/max recomputation in numpy
+  update with rotations     after many deformed frames: every object in exactly one leaf,
+                            boxes the exact unions of their children's and objects' boxes,
+                            subtree sizes and the node count consistent, sorted leaves,
+                            and closest hits equal to brute force outside the tie bands;
+                            on hand-built trees, the rotation that lowers the cost fires
+                            (and none fires when none helps), and a two-cluster leaf is
+                            split between the clusters by the approximate SAH
 """
 
 import numpy as np
@@ -395,3 +402,166 @@ def test_refit_exact(oracle):
     # topology is untouched
     assert np.array_equal(ref["child"], nodes["child"])
     assert np.array_equal(ref["children"], nodes["children"])
+
+
+def _rot_check(oracle, st, v, i):
+    """Structural invariants of an updated tree; returns the reachable record count."""
+    nodes, ids = st.nodes, st.ids
+    tb_lo = v[i].min(1)
+    tb_hi = v[i].max(1)
+    seen = np.zeros(len(i), np.int32)
+    count = [0]
+
+    def walk(n):
+        count[0] += 1
+        if nodes["children"][n]:
+            a, c = int(nodes["child"][n]), int(nodes["children"][n])
+            leaf_ids = ids[a:a + c]
+            assert np.all(np.diff(leaf_ids) > 0), "leaf ids sorted"
+            seen[leaf_ids] += 1
+            lo, hi = tb_lo[leaf_ids].min(0), tb_hi[leaf_ids].max(0)
+            size = 1
+        else:
+            c = int(nodes["child"][n])
+            l0, h0, s0 = walk(c)
+            l1, h1, s1 = walk(c + 1)
+            lo, hi, size = np.minimum(l0, l1), np.maximum(h0, h1), 1 + s0 + s1
+        assert np.array_equal(nodes["lo"][n], lo) and np.array_equal(nodes["hi"][n], hi), n
+        # The reference recomputes a split leaf's size but not its ancestors' (rotateNode,
+        # DynBVH.cc:3057-3069), so stored sizes may lag the true ones, never exceed them.
+        assert st.ssize[n] <= size, n
+        return lo, hi, size
+
+    walk(0)
+    assert np.all(seen == 1), "every object in exactly one leaf"
+    assert count[0] == st.next_free.value, "allocated records are all reachable"
+    return count[0]
+
+
+@pytest.mark.parametrize("mesh", ["heightfield", "soup"])
+def test_update_with_rotations_stays_valid(oracle, mesh):
+    if mesh == "heightfield":
+        v, i = scenes.heightfield(60, 40, seed=2)
+        amp = 0.05
+    else:
+        v, i = scenes.soup(1500, seed=4)
+        amp = 0.4
+    nodes, ids, nn = oracle.build_bvh(v, i)
+    st = oracle.RotState(nodes, ids, nn)
+    rays = scenes.random_rays(3000, 8, (-1.3, -1.3, -1.3), (1.3, 1.3, 1.3))
+    changed = 0
+    for f in range(6):
+        vd = v.copy()
+        vd[:, 1] += (amp * np.sin(4 * v[:, 0] + 0.9 * f)).astype(np.float32)
+        before = st.nodes.copy()
+        st.update(vd, i)
+        changed += int((st.nodes["child"] != before["child"]).sum())
+        _rot_check(oracle, st, vd, i)
+        g = oracle.trace(vd, i, st.nodes, st.ids, rays, packet_size=1)
+        b = oracle.bruteforce(vd, i, rays)
+        h1, h2, real = tie_classes(b["prim"], b["t"], g["prim"], g["t"])
+        assert real == 0, (f, h1, h2, real)
+    assert changed > 0, "rotations happened"
+
+
+def _tri(c, r=0.01):
+    """A small triangle around centre c."""
+    c = np.asarray(c, np.float32)
+    return np.array([c + [-r, -r, 0], c + [r, -r, 0], c + [0, r, 0]], np.float32)
+
+
+def _manual_tree(oracle, centres, leaves, structure):
+    """Mesh of small triangles at `centres` and a tree given as nested tuples of leaf
+    indices (`leaves[k]` = object ids of leaf k); boxes are filled by a refit."""
+    v = np.concatenate([_tri(c) for c in centres])
+    i = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    T = len(i)
+    nodes = np.zeros(2 * T, oracle.NODE_DTYPE)
+    nodes["lo"] = np.float32(3.4028235e38)
+    nodes["hi"] = np.float32(-3.4028235e38)
+    ids = np.concatenate([np.asarray(l, np.int32) for l in leaves])
+    starts = np.concatenate([[0], np.cumsum([len(l) for l in leaves])[:-1]])
+    nxt = [1]
+
+    def place(rec, sub):
+        if isinstance(sub, int):
+            nodes["child"][rec] = starts[sub]
+            nodes["children"][rec] = len(leaves[sub])
+            return
+        c = nxt[0]
+        nxt[0] += 2
+        nodes["child"][rec] = c
+        nodes["children"][rec] = 0
+        place(c, sub[0])
+        place(c + 1, sub[1])
+
+    place(0, structure)
+    oracle.refit(v, i, nodes, ids)
+    return v, i, nodes, ids, nxt[0]
+
+
+def _shape(nodes, ids, n=0):
+    if nodes["children"][n]:
+        a, c = int(nodes["child"][n]), int(nodes["children"][n])
+        return frozenset(int(x) for x in ids[a:a + c])
+    c = int(nodes["child"][n])
+    return frozenset([_shape(nodes, ids, c), _shape(nodes, ids, c + 1)])
+
+
+def test_rotation_fires_when_it_lowers_the_cost(oracle):
+    # L = (LL, LR) with LL far away at x = 5 and LR next to R at x = 0: exchanging LL and
+    # R (rotation 1) makes L = (R, LR) tight and is the only cheaper configuration.
+    centres = [(5.0, 0, 0), (0.0, 0, 0), (0.1, 0, 0)]
+    v, i, nodes, ids, nf = _manual_tree(oracle, centres, [[0], [1], [2]], ((0, 1), 2))
+    st = oracle.RotState(nodes, ids, nf)
+    st.update(v, i)
+    got = _shape(st.nodes, st.ids)
+    want = frozenset([frozenset([0]), frozenset([frozenset([1]), frozenset([2])])])
+    assert got == want
+    _rot_check(oracle, st, v, i)
+    # already the cheaper tree: a second update leaves the topology alone
+    before = _shape(st.nodes, st.ids)
+    st.update(v, i)
+    assert _shape(st.nodes, st.ids) == before
+
+
+def test_no_rotation_when_none_helps(oracle):
+    centres = [(0.0, 0, 0), (0.1, 0, 0), (5.0, 0, 0), (5.1, 0, 0)]
+    v, i, nodes, ids, nf = _manual_tree(oracle, centres, [[0], [1], [2], [3]], ((0, 1), (2, 3)))
+    st = oracle.RotState(nodes, ids, nf)
+    want = _shape(st.nodes, st.ids)
+    st.update(v, i)
+    assert _shape(st.nodes, st.ids) == want
+
+
+def test_leaf_split_by_approximate_sah(oracle):
+    # One 8-object leaf (the root's left child) made of two clusters along x: the
+    # approximate SAH splits it between the clusters during the update.
+    centres = [(0.00, 0, 0), (0.03, 0, 0), (0.06, 0, 0), (0.09, 0, 0),
+               (2.00, 0, 0), (2.03, 0, 0), (2.06, 0, 0), (2.09, 0, 0), (9.0, 9.0, 0)]
+    v, i, nodes, ids, nf = _manual_tree(oracle, centres,
+                                        [[0, 1, 2, 3, 4, 5, 6, 7], [8]], (0, 1))
+    st = oracle.RotState(nodes, ids, nf)
+    st.update(v, i)
+    shape = _shape(st.nodes, st.ids)
+    a, b = frozenset([0, 1, 2, 3]), frozenset([4, 5, 6, 7])
+
+    def flat(sh):
+        if all(isinstance(x, int) for x in sh):
+            return frozenset(sh)
+        out = set()
+        for p in sh:
+            out |= flat(p)
+        return frozenset(out)
+
+    def contains_split(sh):
+        """A node whose two subtrees hold exactly the two clusters."""
+        if all(isinstance(x, int) for x in sh):
+            return False
+        if {flat(p) for p in sh} == {a, b}:
+            return True
+        return any(contains_split(p) for p in sh)
+
+    assert st.next_free.value > nf, "new records were allocated"
+    assert contains_split(shape), shape
+    _rot_check(oracle, st, v, i)
