@@ -87,7 +87,6 @@ typedef struct mh_hits {
  * and visit order. */
 #define MH_LAYOUT_PAIR 0 /* 64-byte nodes holding both child boxes */
 #define MH_LAYOUT_REF 1  /* the reference's 32-byte BVHNode records (frames only) */
-#define MH_LAYOUT_QUAD 2 /* 128-byte nodes holding the four grandchild boxes */
 
 /* Trace options; NULL means the defaults given in brackets. */
 typedef struct mh_trace_opts {
@@ -99,11 +98,8 @@ typedef struct mh_trace_opts {
   int stack_lds;
   /* [0] persistent blocks per CU (0 = autotuned). */
   int blocks_per_cu;
-  /* [NULL] optional device int64[2]: box tests, triangle tests (accumulated) of the
-   * binary traversal (counting runs use the PAIR layout). */
+  /* [NULL] optional device int64[2]: box tests, triangle tests (accumulated). */
   int64_t* counters;
-  /* [MH_LAYOUT_PAIR] single-ray node layout: MH_LAYOUT_PAIR or MH_LAYOUT_QUAD. */
-  int layout;
 } mh_trace_opts;
 
 /* Build options. */
@@ -156,10 +152,18 @@ int mh_scene_stats(const mh_scene* scene, uint32_t* n_nodes, uint32_t* n_inner, 
                    size_t* device_bytes);
 
 /* Per-frame update — AccelerationStructure::update/rebuild without group change
- * (DynBVH.cc:1574-1581, 2012-2123): replace the vertex positions (same topology),
- * then refit the tree bottom-up on the GPU. set_vertices copies from host memory
- * (on_device = 0) or from device memory (on_device = 1) on `stream`. */
+ * (DynBVH.cc:1574-1581, 2012-2123): replace the vertex positions (same topology), then
+ * update the tree on the GPU. set_vertices copies from host memory (on_device = 0) or
+ * from device memory (on_device = 1) on `stream`.
+ * mh_scene_update is DynBVH::update as the reference builds it (TREE_ROT set,
+ * DynBVH.h:377): refit with tree rotations and approximate-SAH leaf splits
+ * (updateBounds<false> + rotateNode, DynBVH.cc:2078-2123, 3051-3372); the tree's
+ * topology changes from frame to frame exactly as the reference's (record numbering of
+ * split leaves may differ). It synchronises `stream` once (level bookkeeping).
+ * mh_scene_refit is the bounds-only refit (updateBounds<false> without rotations): the
+ * topology never changes; asynchronous on `stream`. */
 int mh_scene_set_vertices(mh_scene* scene, const float* verts, int on_device, void* stream);
+int mh_scene_update(mh_scene* scene, void* stream);
 int mh_scene_refit(mh_scene* scene, void* stream);
 /* Device pointer to the scene's vertex array (3 floats per vertex), for callers that
  * animate in place; call mh_scene_refit afterwards. */
@@ -199,9 +203,8 @@ typedef struct mh_launch_config {
   int stack_lds;     /* per-lane LDS stack ring: 8, 16 or 32 entries */
   int block;         /* lanes per workgroup: 64, 128 or 256 */
   int blocks_per_cu; /* persistent workgroups per compute unit */
-  int refill;        /* idle lanes of a wave that trigger a refill, 1..63 (PAIR and REF);
-                        64 = whole-wave chunks (a wave traces `chunk` consecutive rays 64
-                        at a time) */
+  int refill;        /* idle lanes of a wave that trigger a refill, 1..63; 64 = whole-wave
+                        chunks (a wave traces `chunk` consecutive rays 64 at a time) */
   int chunk;         /* rays a wave takes per grab from the shared counter, a multiple of 64
                         (its first grab is assigned statically); 0 = static stride over the
                         rays with no counter (whole-wave mode only) */

@@ -47,7 +47,7 @@ class Hits(C.Structure):
 
 class TraceOpts(C.Structure):
     _fields_ = [("packet_size", C.c_int), ("stack_lds", C.c_int), ("blocks_per_cu", C.c_int),
-                ("counters", C.c_void_p), ("layout", C.c_int)]
+                ("counters", C.c_void_p)]
 
 
 class Batch(C.Structure):
@@ -70,7 +70,6 @@ class LaunchConfig(C.Structure):
 
 MH_LAYOUT_PAIR = 0
 MH_LAYOUT_REF = 1
-MH_LAYOUT_QUAD = 2
 
 
 class RayBuffers(C.Structure):
@@ -134,6 +133,7 @@ _SIGS = {
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_size_t)]),
     "mh_scene_set_vertices": (C.c_int, [_P, _P, C.c_int, _P]),
     "mh_scene_refit": (C.c_int, [_P, _P]),
+    "mh_scene_update": (C.c_int, [_P, _P]),
     "mh_scene_vertices_device": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "mh_scene_displace_sine": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P]),
     "mh_trace": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,

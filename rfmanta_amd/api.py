@@ -131,13 +131,12 @@ class DeviceHits:
 
 
 def trace_opts(packet_size: int = 1, stack_lds: int = 0, blocks_per_cu: int = 0,
-               counters: DeviceArray | None = None, layout: int = 0) -> L.TraceOpts:
+               counters: DeviceArray | None = None) -> L.TraceOpts:
     o = L.TraceOpts()
     o.packet_size = packet_size
     o.stack_lds = stack_lds
     o.blocks_per_cu = blocks_per_cu
     o.counters = counters.ptr if counters is not None else None
-    o.layout = layout
     return o
 
 
@@ -223,7 +222,12 @@ class Scene:
         self.verts = v.reshape(-1, 3)
 
     def refit(self, stream=None) -> None:
+        """Bounds-only refit (no rotations)."""
         L.check(L.load().mh_scene_refit(self.h, stream))
+
+    def update(self, stream=None) -> None:
+        """DynBVH::update: refit with tree rotations and leaf splits (synchronises)."""
+        L.check(L.load().mh_scene_update(self.h, stream))
 
     def displace_sine(self, amp: float, freq: float, phase: float, stream=None) -> None:
         L.check(L.load().mh_scene_displace_sine(self.h, amp, freq, phase, stream))
@@ -244,7 +248,7 @@ class Scene:
                                   C.byref(opts) if opts is not None else None, stream))
 
     def trace_host(self, rays: dict, any_hit: bool = False, packet_size: int = 1,
-                   stack_lds: int = 0, layout: int = 0) -> dict:
+                   stack_lds: int = 0) -> dict:
         n = len(rays["ox"])
         arr = {k: np.ascontiguousarray(rays[k], np.float32) for k in RAY_FIELDS}
         m = rays.get("masked")
@@ -256,7 +260,7 @@ class Scene:
         r = L.Rays(*[arr[k].ctypes.data for k in RAY_FIELDS], L.ptr(m))
         h = L.Hits(out["prim"].ctypes.data, out["t"].ctypes.data, out["u"].ctypes.data,
                    out["v"].ctypes.data, out["occluded"].ctypes.data if any_hit else None)
-        o = trace_opts(packet_size, stack_lds, layout=layout)
+        o = trace_opts(packet_size, stack_lds)
         L.check(L.load().mh_trace_host(self.h, C.byref(r), C.byref(h), n,
                                        L.MH_ANY_HIT if any_hit else L.MH_CLOSEST_HIT, C.byref(o)))
         return out

@@ -10,12 +10,9 @@ int launch_trace_batch(const DevScene& sc, const BatchLaunch& bl, void* stream) 
     return (int)hipErrorInvalidValue;
   if (bl.kinds < kKindMixed || bl.kinds > kKindAny) return (int)hipErrorInvalidValue;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (sc.prim_kind == 1) {
-    if (bl.layout == kLayoutQuad) return launch_batch_wald_quad(sc, bl, s);
+  if (sc.prim_kind == 1)
     return bl.layout == kLayoutRef ? launch_batch_wald_ref(sc, bl, s)
                                    : launch_batch_wald_pair(sc, bl, s);
-  }
-  if (bl.layout == kLayoutQuad) return launch_batch_ks_quad(sc, bl, s);
   return bl.layout == kLayoutRef ? launch_batch_ks_ref(sc, bl, s)
                                  : launch_batch_ks_pair(sc, bl, s);
 }

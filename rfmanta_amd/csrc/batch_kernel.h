@@ -17,11 +17,10 @@
 // The last wave to leave resets the counters, so consecutive launches (and graph
 // replays) need no clearing pass.
 //
-// Three node layouts, all giving the reference's one-ray results and visit order
+// Two node layouts, both giving the reference's one-ray results and visit order
 // (DynBVH::intersectNode on a one-ray packet, Model/Groups/DynBVH.cc:274-404; the
-// traversal loops are traverse.h's): PAIR (64-byte records with both child boxes), REF
-// (the reference's 32-byte records) and QUAD (128-byte records with four grandchild
-// boxes, whole-wave schedules only).
+// traversal loops are traverse.h's): PAIR (64-byte records with both child boxes) and REF
+// (the reference's 32-byte records).
 // Triangle tests and arithmetic are those of trace.hip (shared header).
 //
 // A table whose batches are all closest-hit (or all any-hit) runs a kernel specialised to
@@ -349,12 +348,7 @@ int launch_k(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
 // refill 64: whole-wave chunks (chunk 0: static stride); otherwise per-lane refill.
 template <int PRIM, int LAYOUT, int BLOCK>
 int launch_s(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
-  if (bl.refill < 64) {
-    // The per-lane state machine steps PAIR or REF nodes; QUAD runs whole-wave schedules.
-    if (LAYOUT == kLayoutQuad) return (int)hipErrorInvalidValue;
-    return launch_w<PRIM, LAYOUT == kLayoutQuad ? kLayoutPair : LAYOUT, BLOCK, kModeLane,
-                    kKindMixed>(sc, bl, s);
-  }
+  if (bl.refill < 64) return launch_w<PRIM, LAYOUT, BLOCK, kModeLane, kKindMixed>(sc, bl, s);
   return bl.chunk == 0 ? launch_k<PRIM, LAYOUT, BLOCK, kModeWaveStatic>(sc, bl, s)
                        : launch_k<PRIM, LAYOUT, BLOCK, kModeWave>(sc, bl, s);
 }
@@ -378,7 +372,5 @@ int launch_batch_ks_pair(const DevScene& sc, const BatchLaunch& bl, hipStream_t 
 int launch_batch_ks_ref(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 int launch_batch_wald_pair(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 int launch_batch_wald_ref(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
-int launch_batch_ks_quad(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
-int launch_batch_wald_quad(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 
 }  // namespace mh

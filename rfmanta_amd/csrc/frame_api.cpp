@@ -45,10 +45,8 @@ struct mh_frame {
 namespace mh_internal {
 
 int check_launch_config(const mh_launch_config& c) {
-  if (c.layout != MH_LAYOUT_PAIR && c.layout != MH_LAYOUT_REF && c.layout != MH_LAYOUT_QUAD)
+  if (c.layout != MH_LAYOUT_PAIR && c.layout != MH_LAYOUT_REF)
     return fail(MH_E_INVALID, "layout %d", c.layout);
-  if (c.layout == MH_LAYOUT_QUAD && c.refill != 64)
-    return fail(MH_E_INVALID, "the QUAD layout runs whole-wave schedules only (refill 64)");
   if (c.stack_lds != 8 && c.stack_lds != 16 && c.stack_lds != 32)
     return fail(MH_E_INVALID, "stack_lds %d not in {8,16,32}", c.stack_lds);
   if (c.block != 64 && c.block != 128 && c.block != 256)
@@ -118,9 +116,11 @@ int prepare(mh_frame* f, const mh_launch_config& c) {
   const int blocks = c.blocks_per_cu * mh_internal::cu_count();
   const int64_t stride = (int64_t)blocks * c.block;
   const int64_t need = std::max<int64_t>(0, (int64_t)f->scene->stack_bound(c.layout) - c.stack_lds);
+  void* old_spill = f->spill.p;
   if (need > 0) {
     if ((rc = f->spill.reserve((size_t)need * stride * 8))) return rc;
   }
+  if (f->spill.p != old_spill) f->drop_graph();  // a captured launch holds the old area
   if (c.layout != f->cfg.layout || c.stack_lds != f->cfg.stack_lds || c.block != f->cfg.block ||
       c.blocks_per_cu != f->cfg.blocks_per_cu || c.refill != f->cfg.refill ||
       c.chunk != f->cfg.chunk || c.xcd_remap != f->cfg.xcd_remap)
@@ -131,8 +131,20 @@ int prepare(mh_frame* f, const mh_launch_config& c) {
   return MH_OK;
 }
 
+// True when the spill area still covers the scene's deepest stack (a rotating update can
+// deepen the tree after the frame was prepared).
+bool spill_ok(const mh_frame* f) {
+  const int64_t need =
+      std::max<int64_t>(0, (int64_t)f->scene->stack_bound(f->cfg.layout) - f->cfg.stack_lds);
+  return (size_t)need * (size_t)f->spill_stride * 8 <= f->spill.bytes;
+}
+
 int launch(mh_frame* f, void* stream) {
   if (f->total == 0) return MH_OK;
+  if (!spill_ok(f)) {
+    int rc = prepare(f, f->cfg);
+    if (rc) return rc;
+  }
   mh::BatchLaunch bl{};
   bl.table = f->table.as<const mh::BatchTable>();
   bl.nb = f->nb;
@@ -325,6 +337,8 @@ int mh_frame_capture(mh_frame* f, void* stream) {
 int mh_frame_replay(mh_frame* f, void* stream) {
   if (!f) return fail(MH_E_INVALID, "null frame");
   if (!f->exec) return fail(MH_E_STATE, "no captured graph for the current configuration");
+  if (!spill_ok(f))
+    return fail(MH_E_STATE, "the tree deepened since the capture: capture the frame again");
   HIP_OK(hipGraphLaunch(f->exec, reinterpret_cast<hipStream_t>(stream)));
   return MH_OK;
 }
@@ -378,14 +392,13 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
     }
     return MH_OK;
   };
-  // Stage 1: node layout x launch shape (stack ring x workgroup size x workgroups per CU)
-  // jointly with the schedule (static stride, small and large counter chunks).
-  for (int l : {MH_LAYOUT_PAIR, MH_LAYOUT_QUAD})
-    for (int s : {8, 16})
-      for (int b : {64, 128, 256})
-        for (int p : {4, 8, 12, 16, 24})
-          for (int c : {0, 64, 256})
-            if ((rc = trial(mh_launch_config{l, s, b, p, 64, c, 0}))) return rc;
+  // Stage 1: launch shape (stack ring x workgroup size x workgroups per CU) jointly with
+  // the schedule (static stride, small and large counter chunks), PAIR layout.
+  for (int s : {8, 16})
+    for (int b : {64, 128, 256})
+      for (int p : {4, 8, 12, 16, 24})
+        for (int c : {0, 64, 256})
+          if ((rc = trial(mh_launch_config{MH_LAYOUT_PAIR, s, b, p, 64, c, 0}))) return rc;
   // Stage 2: chunk size x XCD remap at the best shape.
   mh_launch_config base = best;
   for (int c : {0, 64, 128, 256, 512, 1024})
@@ -395,18 +408,17 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
       t.xcd_remap = x;
       if ((rc = trial(t))) return rc;
     }
-  // Stage 3: the three node layouts with the deepest LDS ring too.
+  // Stage 3: the reference's 32-byte node layout and the deepest LDS ring.
   base = best;
-  for (int l : {MH_LAYOUT_PAIR, MH_LAYOUT_REF, MH_LAYOUT_QUAD})
+  for (int l : {MH_LAYOUT_PAIR, MH_LAYOUT_REF})
     for (int s : {8, 16, 32}) {
       mh_launch_config t = base;
       t.layout = l;
       t.stack_lds = s;
       if ((rc = trial(t))) return rc;
     }
-  // Stage 4: per-lane refill thresholds (PAIR or REF).
+  // Stage 4: per-lane refill thresholds.
   base = best;
-  if (base.layout == MH_LAYOUT_QUAD) base.layout = MH_LAYOUT_PAIR;
   for (int r : {8, 16, 32})
     for (int c : {64, 256}) {
       mh_launch_config t = base;

@@ -79,11 +79,13 @@ struct mh_scene {
   bool built = false;
   int prim_kind = MH_PRIM_KENSLER_SHIRLEY;
   int root_ref = 0;
-  int quad_root = 0;
-  uint32_t quad_stack = 0;  // stack entries a QUAD traversal can hold at most
+  uint32_t next_free = 0;  // records in use (the reference's nextFree)
   uint32_t n_inner = 0, n_leaves = 0, depth = 0;
 
-  DevBuf refs, pairs, quads, tris, ids, big, d_verts, d_base, d_idx, slot, qslot, leaves, levels;
+  DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, leaves, levels;
+  // rotating update (mh_scene_update): costs, subtree sizes, parents, object boxes and
+  // centroids, depth work arrays, the depth histogram and counters
+  DevBuf costs, ssize, parent, obox, ocen, anc0, dep0, anc1, dep1, hist, counters;
   std::vector<int64_t> level_off;  // inner records of depth d: levels[level_off[d], level_off[d+1])
 
   std::mutex spill_mu;  // guards the per-stream spill areas
@@ -94,19 +96,18 @@ struct mh_scene {
     for (auto& kv : spill) delete kv.second;
   }
   size_t device_bytes() const {
-    return refs.bytes + pairs.bytes + quads.bytes + tris.bytes + ids.bytes + big.bytes +
-           d_verts.bytes + d_base.bytes + d_idx.bytes + slot.bytes + qslot.bytes + leaves.bytes +
-           levels.bytes;
+    return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
+           d_base.bytes + d_idx.bytes + leaves.bytes + levels.bytes + costs.bytes + ssize.bytes +
+           parent.bytes + obox.bytes + ocen.bytes + anc0.bytes + dep0.bytes + anc1.bytes +
+           dep1.bytes + hist.bytes + counters.bytes;
   }
-  // Traversal stack entries a single ray can hold at most with a node layout.
-  uint32_t stack_bound(int layout) const {
-    return layout == MH_LAYOUT_QUAD ? quad_stack : depth;
+  // Traversal stack entries a single ray can hold at most (one push per level).
+  uint32_t stack_bound(int /*layout*/) const {
+    return depth;
   }
   mh::DevScene dev() const {
     mh::DevScene d;
     d.pairs = pairs.as<const mh::PairNode>();
-    d.quads = quads.as<const mh::QuadNode>();
-    d.quad_root = quad_root;
     d.refs = refs.as<const mh::RefNode>();
     d.tris = tris.as<const float>();
     d.obj_ids = ids.as<const int32_t>();

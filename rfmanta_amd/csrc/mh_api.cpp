@@ -81,29 +81,65 @@ using mh_internal::g_lib;
 
 namespace {
 
-// Lays the reference tree out for the device: PairNodes in depth-first preorder of inner
-// records, leaf references, refit links and stack-depth bound, then uploads.
+// Rotation state after DynBVH::rebuild with TREE_ROT: computeCost<true>(0)
+// (DynBVH.cc:3388-3415) and subtree sizes (DynBVH.cc:1971-1996) of the reachable records.
+void rotation_state(const std::vector<mh::RefNode>& nodes, std::vector<float>& costs,
+                    std::vector<uint32_t>& ssize) {
+  costs.assign(nodes.size(), 0.f);
+  ssize.assign(nodes.size(), 0u);
+  auto area = [](const mh::Aabb& b) {  // BBox::computeArea, float arithmetic
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return 2 * (dx * dy + dy * dz + dx * dz);
+  };
+  struct It {
+    int32_t rec;
+    bool done;
+  };
+  std::vector<It> st{{0, false}};
+  while (!st.empty()) {
+    const It it = st.back();
+    st.pop_back();
+    const mh::RefNode& n = nodes[it.rec];
+    if (n.leaf()) {
+      costs[it.rec] = n.children * 1.f;
+      ssize[it.rec] = 1;
+      continue;
+    }
+    if (!it.done) {
+      st.push_back({it.rec, true});
+      st.push_back({n.child + 1, false});
+      st.push_back({n.child, false});
+      continue;
+    }
+    const int l = n.child, r = n.child + 1;
+    costs[it.rec] = 1.f + (area(nodes[l].box) * costs[l] + area(nodes[r].box) * costs[r]) /
+                              area(n.box);
+    ssize[it.rec] = 1 + ssize[l] + ssize[r];
+  }
+}
+
+// Lays the reference tree out for the device: PairNodes indexed by sibling pair, leaf
+// references, the leaf and per-depth lists for refits, the rotation state, then uploads.
 int upload(mh_scene* s) {
   const std::vector<mh::RefNode>& nodes = s->bvh.nodes;
   const uint32_t nrec = (uint32_t)nodes.size();
   const uint32_t T = s->nt;
-  std::vector<int32_t> pair_of(nrec, -1), slot(nrec, -1);
   std::vector<int32_t> leaves;
   std::vector<std::vector<int32_t>> by_depth;  // inner records per depth, preorder
   std::vector<int32_t> big(T, 0);
-  std::vector<int32_t> order;  // inner records, preorder
   struct It {
     int32_t rec;
     uint32_t depth;
   };
   std::vector<It> st;
   st.push_back({0, 0});
-  uint32_t max_depth = 0;
+  uint32_t max_depth = 0, n_inner = 0, max_rec = 0;
   while (!st.empty()) {
     It it = st.back();
     st.pop_back();
     const mh::RefNode& n = nodes[it.rec];
     max_depth = std::max(max_depth, it.depth);
+    max_rec = std::max(max_rec, (uint32_t)it.rec);
     if (n.leaf()) {
       if (n.children < 0) return fail(MH_E_INVALID, "record %d: negative leaf size", it.rec);
       if ((uint32_t)n.child + (uint32_t)n.children > T || n.child < 0)
@@ -114,10 +150,13 @@ int upload(mh_scene* s) {
     }
     if (n.child < 1 || (uint32_t)n.child + 1 >= nrec)
       return fail(MH_E_INVALID, "record %d: child %d outside the node array", it.rec, n.child);
+    if ((n.child & 1) == 0)
+      return fail(MH_E_INVALID, "record %d: child %d is not the first record of a pair "
+                  "(DynBVH allocates sibling records in pairs from 1)", it.rec, n.child);
     if (n.axis > 2) return fail(MH_E_INVALID, "record %d: axis %d", it.rec, (int)n.axis);
-    pair_of[it.rec] = (int32_t)order.size();
-    order.push_back(it.rec);
-    if (order.size() > nrec) return fail(MH_E_INVALID, "node graph is not a tree");
+    if (++n_inner > nrec) return fail(MH_E_INVALID, "node graph is not a tree");
+    if (it.depth >= (uint32_t)mh::kMaxDepth - 1)
+      return fail(MH_E_UNSUPPORTED, "tree deeper than %d levels", mh::kMaxDepth - 1);
     if (by_depth.size() <= it.depth) by_depth.resize(it.depth + 1);
     by_depth[it.depth].push_back(it.rec);
     st.push_back({n.child + 1, it.depth + 1});
@@ -125,127 +164,66 @@ int upload(mh_scene* s) {
   }
   auto encode = [&](int32_t rec) -> int32_t {
     const mh::RefNode& n = nodes[rec];
-    if (!n.leaf()) return pair_of[rec];
+    if (!n.leaf()) return (n.child - 1) >> 1;
     const int c = std::min<int>(n.children, mh::kLeafCountInline);
     return ~((n.child << mh::kLeafCountBits) | c);
   };
-  std::vector<mh::PairNode> pairs(order.size());
-  for (size_t pi = 0; pi < order.size(); ++pi) {
-    const mh::RefNode& n = nodes[order[pi]];
-    mh::PairNode& P = pairs[pi];
-    for (int side = 0; side < 2; ++side) {
-      const mh::RefNode& c = nodes[n.child + side];
-      float* b = P.b + 6 * side;
-      b[0] = c.box.lo[0];
-      b[1] = c.box.hi[0];
-      b[2] = c.box.lo[1];
-      b[3] = c.box.hi[1];
-      b[4] = c.box.lo[2];
-      b[5] = c.box.hi[2];
-      P.ref[side] = encode(n.child + side);
-      slot[n.child + side] = (int32_t)(pi * 2 + side);
-    }
-    P.meta = (int32_t)n.axis | ((n.bits & 1) << 2);
-    P.node = order[pi];
-  }
-  s->root_ref = encode(0);
-  // QuadNodes: one per inner record that is the root or a grandchild slot of a quad
-  // (two binary levels per record), in depth-first order of the quads.
-  std::vector<mh::QuadNode> quads;
-  std::vector<int32_t> quad_of(nrec, -1), qslot(nrec, -1);
-  uint32_t max_wdepth = 0;
-  if (!nodes[0].leaf()) {
-    quad_of[0] = 0;
-    quads.emplace_back();
-    std::vector<It> qs;
-    qs.push_back({0, 0});
-    while (!qs.empty()) {
-      const It it = qs.back();
-      qs.pop_back();
-      max_wdepth = std::max(max_wdepth, it.depth);
-      const int32_t q = quad_of[it.rec];
-      const mh::RefNode& n = nodes[it.rec];
-      mh::QuadNode Q{};
-      Q.node = it.rec;
-      Q.meta = (n.axis & 3) | ((n.bits & 1) << 2);
-      int32_t grand[4] = {-1, -1, -1, -1};
+  const size_t n_pairs = std::max<size_t>(1, nrec / 2);
+  std::vector<mh::PairNode> pairs(n_pairs);
+  for (const auto& lv : by_depth)
+    for (int32_t r : lv) {
+      const mh::RefNode& n = nodes[r];
+      mh::PairNode& P = pairs[(n.child - 1) >> 1];
       for (int side = 0; side < 2; ++side) {
-        const int32_t c = n.child + side;
-        const mh::RefNode& cn = nodes[c];
-        if (cn.leaf()) {
-          grand[2 * side] = c;
-        } else {
-          grand[2 * side] = cn.child;
-          grand[2 * side + 1] = cn.child + 1;
-          Q.meta |= ((cn.axis & 3) | ((cn.bits & 1) << 2)) << (3 + 3 * side);
-        }
+        const mh::RefNode& c = nodes[n.child + side];
+        float* b = P.b + 6 * side;
+        b[0] = c.box.lo[0];
+        b[1] = c.box.hi[0];
+        b[2] = c.box.lo[1];
+        b[3] = c.box.hi[1];
+        b[4] = c.box.lo[2];
+        b[5] = c.box.hi[2];
+        P.ref[side] = encode(n.child + side);
       }
-      for (int k = 0; k < 4; ++k) {
-        const int32_t g = grand[k];
-        if (g < 0) continue;
-        Q.meta |= 1 << (9 + k);
-        const mh::RefNode& gn = nodes[g];
-        float* b = Q.b + 6 * k;
-        b[0] = gn.box.lo[0];
-        b[1] = gn.box.hi[0];
-        b[2] = gn.box.lo[1];
-        b[3] = gn.box.hi[1];
-        b[4] = gn.box.lo[2];
-        b[5] = gn.box.hi[2];
-        qslot[g] = q * 4 + k;
-        if (gn.leaf()) {
-          Q.ref[k] = encode(g);
-        } else {
-          quad_of[g] = (int32_t)quads.size();
-          quads.emplace_back();
-          Q.ref[k] = quad_of[g];
-        }
-      }
-      quads[q] = Q;
-      for (int k = 3; k >= 0; --k)
-        if (grand[k] >= 0 && !nodes[grand[k]].leaf()) qs.push_back({grand[k], it.depth + 1});
+      P.meta = (int32_t)n.axis | ((n.bits & 1) << 2);
+      P.node = r;
     }
-    s->quad_root = 0;
-  } else {
-    s->quad_root = encode(0);
-  }
-  s->quad_stack = 3 * (max_wdepth + 1);
-  s->n_inner = (uint32_t)order.size();
+  s->root_ref = encode(0);
+  s->n_inner = n_inner;
   s->n_leaves = (uint32_t)leaves.size();
   s->depth = max_depth;
+  s->next_free = max_rec + 1;
   std::vector<int32_t> levels;
   s->level_off.assign(1, 0);
   for (auto& v : by_depth) {
     levels.insert(levels.end(), v.begin(), v.end());
     s->level_off.push_back((int64_t)levels.size());
   }
+  std::vector<float> costs;
+  std::vector<uint32_t> ssize;
+  rotation_state(nodes, costs, ssize);
 
   int rc;
   if ((rc = s->refs.reserve(nrec * sizeof(mh::RefNode)))) return rc;
-  if ((rc = s->pairs.reserve(std::max<size_t>(1, pairs.size()) * sizeof(mh::PairNode)))) return rc;
+  if ((rc = s->pairs.reserve(n_pairs * sizeof(mh::PairNode)))) return rc;
   if ((rc = s->tris.reserve((size_t)T * 12 * sizeof(float)))) return rc;
   if ((rc = s->ids.reserve((size_t)T * sizeof(int32_t)))) return rc;
   if ((rc = s->big.reserve((size_t)T * sizeof(int32_t)))) return rc;
-  if ((rc = s->slot.reserve(nrec * sizeof(int32_t)))) return rc;
-  if ((rc = s->qslot.reserve(nrec * sizeof(int32_t)))) return rc;
-  if ((rc = s->quads.reserve(std::max<size_t>(1, quads.size()) * sizeof(mh::QuadNode)))) return rc;
-  if ((rc = s->leaves.reserve(std::max<size_t>(1, leaves.size()) * sizeof(int32_t)))) return rc;
-  if ((rc = s->levels.reserve(std::max<size_t>(1, levels.size()) * sizeof(int32_t)))) return rc;
+  if ((rc = s->leaves.reserve((size_t)std::max(1u, T) * sizeof(int32_t)))) return rc;
+  if ((rc = s->levels.reserve((size_t)std::max(1u, T) * sizeof(int32_t)))) return rc;
+  if ((rc = s->costs.reserve(nrec * sizeof(float)))) return rc;
+  if ((rc = s->ssize.reserve(nrec * sizeof(uint32_t)))) return rc;
   HIP_OK(hipMemcpy(s->refs.p, nodes.data(), nrec * sizeof(mh::RefNode), hipMemcpyHostToDevice));
-  if (!pairs.empty())
-    HIP_OK(hipMemcpy(s->pairs.p, pairs.data(), pairs.size() * sizeof(mh::PairNode),
-                     hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->pairs.p, pairs.data(), n_pairs * sizeof(mh::PairNode),
+                   hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->ids.p, s->bvh.ids.data(), (size_t)T * 4, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->big.p, big.data(), (size_t)T * 4, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->slot.p, slot.data(), nrec * 4, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->qslot.p, qslot.data(), nrec * 4, hipMemcpyHostToDevice));
-  if (!quads.empty())
-    HIP_OK(hipMemcpy(s->quads.p, quads.data(), quads.size() * sizeof(mh::QuadNode),
-                     hipMemcpyHostToDevice));
   if (!leaves.empty())
     HIP_OK(hipMemcpy(s->leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice));
   if (!levels.empty())
     HIP_OK(hipMemcpy(s->levels.p, levels.data(), levels.size() * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->costs.p, costs.data(), nrec * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->ssize.p, ssize.data(), nrec * sizeof(uint32_t), hipMemcpyHostToDevice));
 
   mh::DevRefit r{};
   r.tris = s->tris.as<float>();
@@ -300,10 +278,6 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
   mh::LaunchCfg cfg{};
   cfg.any_hit = any;
   cfg.packet_size = psize;
-  cfg.layout = opts ? opts->layout : MH_LAYOUT_PAIR;
-  if (cfg.layout != MH_LAYOUT_PAIR && cfg.layout != MH_LAYOUT_QUAD)
-    return fail(MH_E_INVALID, "trace layout %d is not MH_LAYOUT_PAIR or MH_LAYOUT_QUAD",
-                cfg.layout);
   cfg.counters = opts ? opts->counters : nullptr;
   if (psize > 1) {
     if (s->depth + 2 > 128)
@@ -316,8 +290,7 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
     cfg.blocks = (int)std::max<int64_t>(1, std::min(want, cap));
     cfg.stack_lds = S;
     cfg.spill_stride = (int64_t)cfg.blocks * 256;
-    const int layout = cfg.counters ? MH_LAYOUT_PAIR : cfg.layout;
-    const int64_t need = std::max<int64_t>(0, (int64_t)s->stack_bound(layout) - S);
+    const int64_t need = std::max<int64_t>(0, (int64_t)s->stack_bound(MH_LAYOUT_PAIR) - S);
     if (need > 0) {
       std::lock_guard<std::mutex> lk(s->spill_mu);
       DevBuf*& b = s->spill[stream];
@@ -504,7 +477,7 @@ int mh_scene_export_dynbvh(const mh_scene* s, int32_t* ids, uint32_t* n_ids, voi
   if (*n_ids < ni || *n_nodes < nn) return fail(MH_E_INVALID, "export arrays too small");
   int rc = ensure_init();
   if (rc) return rc;
-  if (ids) std::memcpy(ids, s->bvh.ids.data(), (size_t)ni * 4);
+  if (ids) HIP_OK(hipMemcpy(ids, s->ids.p, (size_t)ni * 4, hipMemcpyDeviceToHost));
   if (nodes) HIP_OK(hipMemcpy(nodes, s->refs.p, (size_t)nn * 32, hipMemcpyDeviceToHost));
   *n_ids = ni;
   *n_nodes = nn;
@@ -593,9 +566,6 @@ int mh_scene_refit(mh_scene* s, void* stream) {
   r.obj_ids = s->ids.as<int32_t>();
   r.verts = s->d_verts.as<float>();
   r.idx = s->d_idx.as<uint32_t>();
-  r.quads = s->quads.as<mh::QuadNode>();
-  r.slot = s->slot.as<int32_t>();
-  r.qslot = s->qslot.as<int32_t>();
   r.leaves = s->leaves.as<int32_t>();
   r.levels = s->levels.as<int32_t>();
   r.n_leaves = (int32_t)s->n_leaves;
@@ -603,6 +573,88 @@ int mh_scene_refit(mh_scene* s, void* stream) {
   r.prim_kind = s->prim_kind;
   HIP_OK((hipError_t)mh::launch_refit(r, s->level_off.data(), (int)s->level_off.size() - 1,
                                       stream));
+  return MH_OK;
+}
+
+int mh_scene_update(mh_scene* s, void* stream) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  if (!s->built) return fail(MH_E_STATE, "scene has no tree");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const uint32_t T = s->nt;
+  const size_t nrec = s->refs.bytes / sizeof(mh::RefNode);
+  // Work arrays, allocated at the first update.
+  if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->obox.reserve((size_t)T * 24)) ||
+      (rc = s->ocen.reserve((size_t)T * 12)) || (rc = s->anc0.reserve(nrec * 4)) ||
+      (rc = s->dep0.reserve(nrec * 4)) || (rc = s->anc1.reserve(nrec * 4)) ||
+      (rc = s->dep1.reserve(nrec * 4)) || (rc = s->hist.reserve((mh::kMaxDepth + 2) * 4)) ||
+      (rc = s->counters.reserve(16)))
+    return rc;
+  unsigned* nfree = s->counters.as<unsigned>();
+  int* err = reinterpret_cast<int*>(nfree + 1);
+  const unsigned init[2] = {s->next_free, 0u};
+  HIP_OK(hipMemcpyAsync(nfree, init, 8, hipMemcpyHostToDevice, st));
+  mh::DevUpdate u{};
+  u.refs = s->refs.as<mh::RefNode>();
+  u.pairs = s->pairs.as<mh::PairNode>();
+  u.big_count = s->big.as<int32_t>();
+  u.costs = s->costs.as<float>();
+  u.ssize = s->ssize.as<uint32_t>();
+  u.parent = s->parent.as<int32_t>();
+  u.ids = s->ids.as<int32_t>();
+  u.obox = s->obox.as<float>();
+  u.ocen = s->ocen.as<float>();
+  u.verts = s->d_verts.as<float>();
+  u.idx = s->d_idx.as<uint32_t>();
+  u.next_free = nfree;
+  u.err = err;
+  u.n_records = (int32_t)nrec;
+  u.n_objects = (int32_t)T;
+  u.large = (uint32_t)(nrec / 20);  // largeSubtreeSize = nodes.size() / 20 (DynBVH.cc:1776)
+  HIP_OK((hipError_t)mh::launch_obj_bounds(u, st));
+  HIP_OK((hipError_t)mh::launch_update_tree(u, s->leaves.as<int32_t>(), s->n_leaves,
+                                            s->levels.as<int32_t>(), s->level_off.data(),
+                                            (int)s->level_off.size() - 1, st));
+  mh::DevRefit r{};
+  r.tris = s->tris.as<float>();
+  r.obj_ids = s->ids.as<int32_t>();
+  r.verts = s->d_verts.as<float>();
+  r.idx = s->d_idx.as<uint32_t>();
+  r.n_slots = (int32_t)T;
+  r.prim_kind = s->prim_kind;
+  HIP_OK((hipError_t)mh::launch_prim_setup(r, st));
+  // Depth lists of the new topology, for the next refit or update.
+  HIP_OK(hipMemsetAsync(s->hist.p, 0, (mh::kMaxDepth + 2) * 4, st));
+  HIP_OK((hipError_t)mh::launch_depths(u, s->anc0.as<int32_t>(), s->dep0.as<int32_t>(),
+                                       s->anc1.as<int32_t>(), s->dep1.as<int32_t>(),
+                                       s->hist.as<unsigned>(), st));
+  std::vector<unsigned> hist(mh::kMaxDepth + 2);
+  unsigned back[2];
+  HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(back, nfree, 8, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (back[1]) return fail(MH_E_STATE, "leaf splits ran out of node records");
+  const unsigned deepest = hist[mh::kMaxDepth + 1];
+  if (deepest >= (unsigned)mh::kMaxDepth - 1)
+    return fail(MH_E_UNSUPPORTED, "tree deeper than %d levels after rotations", mh::kMaxDepth - 1);
+  s->next_free = back[0];
+  std::vector<unsigned> cursor(mh::kMaxDepth + 1, 0u);
+  s->level_off.assign(1, 0);
+  uint32_t n_inner = 0;
+  for (unsigned d = 0; d < deepest; ++d) {  // inner records lie above the deepest depth
+    cursor[d] = (unsigned)s->level_off.back();
+    s->level_off.push_back(s->level_off.back() + hist[d]);
+    n_inner += hist[d];
+  }
+  cursor[mh::kMaxDepth] = 0;
+  s->n_inner = n_inner;
+  s->n_leaves = hist[mh::kMaxDepth];
+  s->depth = deepest;
+  HIP_OK(hipMemcpyAsync(s->hist.p, cursor.data(), cursor.size() * 4, hipMemcpyHostToDevice, st));
+  HIP_OK((hipError_t)mh::launch_depth_scatter(u, s->dep0.as<int32_t>(), s->hist.as<unsigned>(),
+                                              s->levels.as<int32_t>(), s->leaves.as<int32_t>(),
+                                              st));
   return MH_OK;
 }
 

@@ -2,18 +2,21 @@
 //
 // HBM layout of one scene (all arrays 256-byte aligned, one allocation each):
 //   refs      RefNode[2T]      reference DynBVH records, 32 B (packet traversal, refit)
-//   pairs     PairNode[I]      one 64-B record per inner node holding both child boxes
-//                              (single-ray traversal; one 64-B fetch decides both children)
-//   quads     QuadNode[Q]      one 128-B record per inner node at an even depth below the
-//                              root or a quad child: the boxes of its grandchildren (or of
-//                              a leaf child), so one fetch replaces two binary levels
+//   pairs     PairNode[T]      one 64-B record per pair of sibling records (2p+1, 2p+2):
+//                              both boxes, so one fetch decides both children of a node
+//                              (single-ray traversal). Sibling records are allocated in
+//                              pairs (nextFree += 2 from 1), so record r sits in pair
+//                              (r - 1) / 2 on side (r - 1) & 1, and a node whose children
+//                              are (c, c + 1) refers to pair (c - 1) / 2.
 //   tris      float4[3T]       per object slot (object_ids order), 48 B:
 //                              Kensler-Shirley: p0, e0 = p1-p0, e1 = p0-p2, n = e0 x e1
 //                              Wald: n_u n_v n_d k | b_nu b_nv b_d n_k | c_nu c_nv c_d 0
 //   obj_ids   int32[T]         object_ids
 //   big_count int32[T]         leaf size at a leaf's first slot when it is >= 15
 //   verts     float[3V], idx uint32[3T], base_verts float[3V]    mesh (refit, animation)
-//   slot, qslot int32[2T], leaves int32[L], levels int32[I]           refit links
+//   leaves int32[L], levels int32[I]   leaf records; inner records grouped by depth
+//   costs float[2T], ssize uint32[2T], parent int32[2T], obj boxes/centroids   rotating
+//                              update state (allocated at the first mh_scene_update)
 #pragma once
 
 #include <cstdint>
@@ -37,29 +40,14 @@ struct alignas(16) PairNode {
 };
 static_assert(sizeof(PairNode) == 64, "pair node is 64 bytes");
 
-// Two binary levels in one record. For an inner record N with children A = child and
-// B = child + 1, slots 0-1 hold A's children (or A itself in slot 0 when A is a leaf) and
-// slots 2-3 hold B's children (or B). A slot's reference is a quad index (>= 0) or a leaf
-// code as in PairNode. meta: bits 0-1 N's split axis, bit 2 N's isLeftCheaper, bits 3-4
-// and 5 the same for A, bits 6-7 and 8 for B, bits 9-12 slot in use.
-struct alignas(16) QuadNode {
-  float b[24];  // per slot: lo.x hi.x lo.y hi.y lo.z hi.z
-  int32_t ref[4];
-  int32_t meta;
-  int32_t node;  // reference record id of N
-  int32_t pad[2];
-};
-static_assert(sizeof(QuadNode) == 128, "quad node is 128 bytes");
 
 struct DevScene {
   const PairNode* pairs;
-  const QuadNode* quads;
   const RefNode* refs;
   const float* tris;  // 12 floats per slot
   const int32_t* obj_ids;
   const int32_t* big_count;
   int32_t root_ref;   // pair-encoded reference of record 0
-  int32_t quad_root;  // quad-encoded reference of record 0
   int32_t prim_kind;  // 0 KS, 1 Wald
 };
 
@@ -76,7 +64,6 @@ struct DevHits {
 
 struct LaunchCfg {
   int any_hit;
-  int layout;         // kLayoutPair or kLayoutQuad (single-ray)
   int packet_size;    // 1 = single-ray kernel
   int stack_lds;      // 8/16/32 (single-ray)
   int blocks;         // grid size in blocks of 256
@@ -85,10 +72,10 @@ struct LaunchCfg {
   int64_t spill_stride;
 };
 
-// Node layouts of the single-ray traversal (MH_LAYOUT_*).
+// Node layouts of the single-ray traversal (MH_LAYOUT_*). A non-root record r's box is
+// also held by PairNode (r - 1) / 2, side (r - 1) & 1.
 constexpr int kLayoutPair = 0;  // 64-byte PairNodes, both child boxes per fetch
 constexpr int kLayoutRef = 1;   // 32-byte reference records, own box tested at visit
-constexpr int kLayoutQuad = 2;  // 128-byte QuadNodes, four grandchild boxes per fetch
 
 // One ray batch of a persistent launch: rays [start, start + count) of the launch's
 // global numbering are rays [0, count) of this batch.
@@ -139,15 +126,45 @@ struct DevRefit {
   const int32_t* obj_ids;
   const float* verts;
   const uint32_t* idx;
-  QuadNode* quads;
-  const int32_t* slot;    // pair index * 2 + side for a non-root record, -1 for the root
-  const int32_t* qslot;   // quad index * 4 + slot for a record held by a quad, else -1
   const int32_t* leaves;  // leaf record ids
   const int32_t* levels;  // inner record ids grouped by depth (level_off gives the groups)
   int32_t n_leaves;
   int32_t n_slots;        // T
   int32_t prim_kind;
 };
+
+// rotate.hip: DynBVH::update with tree rotations and leaf splits.
+constexpr int kMaxDepth = 256;  // deepest tree the level lists hold
+
+struct DevUpdate {
+  RefNode* refs;
+  PairNode* pairs;
+  int32_t* big_count;
+  float* costs;          // DynBVH::costs (Real), per record
+  uint32_t* ssize;       // DynBVH::subtree_size, per record
+  int32_t* parent;       // parent links of records built by leaf splits
+  int32_t* ids;          // object_ids (leaf splits permute ranges)
+  float* obox;           // per object: lo xyz, hi xyz
+  float* ocen;           // per object: centroid xyz
+  const float* verts;
+  const uint32_t* idx;
+  unsigned* next_free;   // records in use (nextFree)
+  int* err;              // set when the record array is full
+  int32_t n_records;     // capacity, 2T
+  int32_t n_objects;     // T
+  uint32_t large;        // largeSubtreeSize
+};
+int launch_obj_bounds(const DevUpdate& u, void* stream);
+// Leaf pass, one launch per depth (deepest first), then the PairNodes.
+int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
+                       const int32_t* levels, const int64_t* level_off, int n_levels,
+                       void* stream);
+// Depth of every record in use; hist[d] inner records at depth d, hist[kMaxDepth] leaves,
+// hist[kMaxDepth + 1] the deepest depth (hist cleared by the caller). Depths in dep0.
+int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc1, int32_t* dep1,
+                  unsigned* hist, void* stream);
+int launch_depth_scatter(const DevUpdate& u, const int32_t* dep, unsigned* cursor,
+                         int32_t* levels, int32_t* leaves, void* stream);
 
 // refit.hip
 int launch_prim_setup(const DevRefit& r, void* stream);  // tris from verts (all slots)

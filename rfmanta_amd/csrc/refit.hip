@@ -7,9 +7,8 @@
 //     BBox::extendByBox, Model/Groups/Mesh.cc:228-237, DynBVH.cc:2091-2097);
 //   k_refit_level: one launch per depth, deepest first; each lane unions the two child
 //     boxes of one inner record (DynBVH.cc:2110-2116).
-// Every box is stored in the reference record, in the parent's PairNode and, for records
-// held by a QuadNode slot, in that slot. Kernel
-// boundaries on the stream order the levels, so no cross-workgroup synchronisation is
+// Every box is stored in the reference record and in its PairNode. Kernel boundaries on
+// the stream order the levels, so no cross-workgroup synchronisation is
 // needed (agent-scope acquire/release would write back and invalidate the per-XCD L2s).
 // Box arithmetic is exact (min/max of the same fp32 values), so the result is
 // bit-identical to a serial recomputation.
@@ -142,19 +141,8 @@ __device__ __forceinline__ void store_box(const DevRefit& r, int rec, const floa
   nd->box.hi[0] = hi[0];
   nd->box.hi[1] = hi[1];
   nd->box.hi[2] = hi[2];
-  const int qcode = r.qslot[rec];
-  if (qcode >= 0) {
-    float* b = r.quads[qcode >> 2].b + 6 * (qcode & 3);
-    b[0] = lo[0];
-    b[1] = hi[0];
-    b[2] = lo[1];
-    b[3] = hi[1];
-    b[4] = lo[2];
-    b[5] = hi[2];
-  }
-  const int code = r.slot[rec];
-  if (code >= 0) {
-    float* b = r.pairs[code >> 1].b + 6 * (code & 1);
+  if (rec > 0) {  // record rec sits in PairNode (rec - 1) / 2 on side (rec - 1) & 1
+    float* b = r.pairs[(rec - 1) >> 1].b + 6 * ((rec - 1) & 1);
     b[0] = lo[0];
     b[1] = hi[0];
     b[2] = lo[1];

@@ -1,0 +1,697 @@
+// Rotating update for gfx950: DynBVH::update with TREE_ROT set (Model/Groups/DynBVH.h:377),
+// the reference's default, on the device.
+//
+// The reference refits in post-order and calls rotateNode on every inner node after its
+// subtree is done (updateBounds<false>, DynBVH.cc:2078-2123). rotateNode first splits
+// leaf children with the approximate SAH build (DynBVH.cc:3057-3069 -> build with
+// useApproximateBuild, partitionApproxSAH DynBVH.cc:2125-2275), then picks the cheapest
+// of six rotations by its cost model and swaps records, and re-guesses split axes
+// (DynBVH.cc:3051-3372). A node only touches records of its own subtree, so all nodes of
+// one depth (depth taken at the start of the frame) run in parallel, deepest first, one
+// thread per node:
+//   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030)
+//   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097)
+//   k_upd_level      one launch per depth: union of the child boxes, then rotateNode
+//   k_rebuild_pairs  PairNodes and big leaf counts from the reference records
+//   k_parents, k_jump, k_depth_hist, k_depth_scatter   depth of every record for the
+//                    next frame's level lists (pointer jumping, then bucketing)
+// Arithmetic is the reference's: Real is float, BBox::computeArea is float arithmetic
+// returned as double, min/max are the select forms, -ffp-contract=off. New record pairs
+// are taken with an atomic counter, so their numbering differs from the reference's
+// serial order while the tree is the same.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "mh_device.h"
+
+namespace mh {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kCIsec = 1.f;  // BVH_C_isec (DynBVH.cc:29)
+constexpr float kCTrav = 1.f;  // BVH_C_trav (DynBVH.cc:30)
+constexpr int kSamples = 16;   // BVH_num_samples (DynBVH.cc:34)
+constexpr uint8_t kPending = 3;  // axis value of a record whose subtree is still to build
+
+struct Box {
+  float lo[3], hi[3];
+};
+
+__device__ __forceinline__ float mn(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float mx(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ void reset(Box& b) {
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = FLT_MAX;
+    b.hi[k] = -FLT_MAX;
+  }
+}
+__device__ __forceinline__ void extend(Box& b, const Box& o) {
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = mn(b.lo[k], o.lo[k]);
+    b.hi[k] = mx(b.hi[k], o.hi[k]);
+  }
+}
+__device__ __forceinline__ void extend_pt(Box& b, const float* p) {
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = mn(b.lo[k], p[k]);
+    b.hi[k] = mx(b.hi[k], p[k]);
+  }
+}
+// BBox::computeArea (Core/Geometry/BBox.h:117-121): float arithmetic.
+__device__ __forceinline__ float area(const Box& b) {
+  const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+  return 2 * (dx * dy + dy * dz + dx * dz);
+}
+__device__ __forceinline__ Box rbox(const RefNode& n) {
+  Box b;
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = n.box.lo[k];
+    b.hi[k] = n.box.hi[k];
+  }
+  return b;
+}
+__device__ __forceinline__ void set_box(RefNode& n, const Box& b) {
+  for (int k = 0; k < 3; ++k) {
+    n.box.lo[k] = b.lo[k];
+    n.box.hi[k] = b.hi[k];
+  }
+}
+__device__ __forceinline__ Box obj_box(const DevUpdate& u, int obj) {
+  Box b;
+  const float* p = u.obox + 6 * (size_t)obj;
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = p[k];
+    b.hi[k] = p[3 + k];
+  }
+  return b;
+}
+__device__ __forceinline__ float obj_cen(const DevUpdate& u, int obj, int axis) {
+  return u.ocen[3 * (size_t)obj + axis];
+}
+// Vector::indexOfMaxComponent (Core/Geometry/Vector.h:382-393)
+__device__ __forceinline__ int index_of_max(float a, float b, float c) {
+  if (a > b) return a > c ? 0 : 2;
+  return b > c ? 1 : 2;
+}
+// BBox::longestAxis (Core/Geometry/BBox.h:123-129)
+__device__ __forceinline__ int longest_axis(const Box& b) {
+  return index_of_max(b.hi[0] - b.lo[0], b.hi[1] - b.lo[1], b.hi[2] - b.lo[2]);
+}
+
+struct Partition {
+  Box l, r;
+  int axis, split;
+};
+
+// partition2Objs (DynBVH.cc:2277-2308)
+__device__ Partition partition2(const DevUpdate& u, int node, int b) {
+  Partition p;
+  reset(p.l);
+  reset(p.r);
+  int32_t* ids = u.ids;
+  const float nodeArea = area(rbox(u.refs[node]));
+  const float leftArea = area(obj_box(u, ids[b]));
+  const float rightArea = area(obj_box(u, ids[b + 1]));
+  const float leafCost = kCIsec * 2;
+  const float newCost = kCTrav + (leftArea * kCIsec + rightArea * kCIsec) / nodeArea;
+  if (newCost < leafCost) {
+    const Box nb = rbox(u.refs[node]);
+    const int axis = index_of_max(nb.hi[0] - nb.lo[0], nb.hi[1] - nb.lo[1], nb.hi[2] - nb.lo[2]);
+    p.axis = axis;
+    p.split = b + 1;
+    if (obj_box(u, ids[b]).lo[axis] > obj_box(u, ids[b + 1]).lo[axis]) {
+      const int t = ids[b];
+      ids[b] = ids[b + 1];
+      ids[b + 1] = t;
+    }
+    p.l = obj_box(u, ids[b]);
+    p.r = obj_box(u, ids[b + 1]);
+    return p;
+  }
+  p.axis = -1;
+  p.split = 0;
+  return p;
+}
+
+// partitionObjects (DynBVH.cc:1369-1396)
+__device__ int partition_objects(const DevUpdate& u, int b, int e, int axis, float position) {
+  int32_t* ids = u.ids;
+  int first = b - 1, last = e;
+  for (;;) {
+    for (++first; first < last; first++)
+      if (obj_cen(u, ids[first], axis) >= position) break;
+    for (--last; first < last; --last)
+      if (obj_cen(u, ids[last], axis) < position) break;
+    if (first < last) {
+      const int t = ids[first];
+      ids[first] = ids[last];
+      ids[last] = t;
+    } else {
+      return first;
+    }
+  }
+}
+
+// partitionApproxSAH (DynBVH.cc:2125-2275); LONGEST_AXIS is defined (DynBVH.cc:39), so
+// only the centroid box's longest axis is binned.
+__device__ Partition partition_approx(const DevUpdate& u, int node, int b, int e) {
+  Partition p;
+  reset(p.l);
+  reset(p.r);
+  const int n = e - b;
+  if (n == 1) {
+    p.axis = -1;
+    p.split = -1;
+    return p;
+  }
+  if (n == 2) return partition2(u, node, b);
+  const int32_t* ids = u.ids;
+  Box cb;
+  reset(cb);
+  for (int i = b; i < e; ++i) {
+    const float c[3] = {obj_cen(u, ids[i], 0), obj_cen(u, ids[i], 1), obj_cen(u, ids[i], 2)};
+    extend_pt(cb, c);
+  }
+  const float inv_parent_area = (float)(1.0 / (double)area(rbox(u.refs[node])));
+  float best_cost = n * kCIsec;
+  int best_axis = -1;
+  float best_pos = FLT_MAX;
+  const int axis = longest_axis(cb);
+  const float width = cb.hi[axis] - cb.lo[axis];
+  const float scale = ((float)kSamples / width) * .999f;
+  Box bins[kSamples];
+  int counts[kSamples];
+  for (int i = 0; i < kSamples; ++i) {
+    reset(bins[i]);
+    counts[i] = 0;
+  }
+  for (int i = b; i < e; ++i) {
+    const float f = (obj_cen(u, ids[i], axis) - cb.lo[axis]) * scale;
+    int which = (f != f) ? 0 : (int)f;
+    which = which < 0 ? 0 : (which > kSamples - 1 ? kSamples - 1 : which);
+    counts[which]++;
+    extend(bins[which], obj_box(u, ids[i]));
+  }
+  float left_areas[kSamples], right_areas[kSamples];
+  int left_counts[kSamples], right_counts[kSamples];
+  Box lb, rb;
+  reset(lb);
+  reset(rb);
+  int nl = 0, nr = n;
+  for (int i = 0; i < kSamples; ++i) {
+    nl += counts[i];
+    nr -= counts[i];
+    extend(lb, bins[i]);
+    left_counts[i] = nl;
+    right_counts[i] = nr;
+    left_areas[i] = area(lb);
+  }
+  for (int i = kSamples - 1; i >= 0; --i) {
+    right_areas[i] = (i == kSamples - 1) ? FLT_MAX : area(rb);
+    extend(rb, bins[i]);
+  }
+  for (int i = 0; i < kSamples - 1; ++i) {
+    const float cost = (left_areas[i] * left_counts[i] + right_areas[i] * right_counts[i]) *
+                           inv_parent_area * kCIsec + kCTrav;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_axis = axis;
+      best_pos = cb.lo[axis] + width * (static_cast<float>(i + 1) / kSamples);
+    }
+  }
+  p.axis = best_axis;
+  if (p.axis != -1) {
+    int middle = partition_objects(u, b, e, p.axis, best_pos);
+    if (middle == b || middle == e) {
+      if (n < 3) {
+        p.axis = -1;
+        p.split = 0;
+        return p;
+      }
+      middle = b + n / 2;
+    }
+    for (int i = b; i < middle; ++i) extend(p.l, obj_box(u, ids[i]));
+    for (int i = middle; i < e; ++i) extend(p.r, obj_box(u, ids[i]));
+    p.split = middle;
+    return p;
+  }
+  p.split = 0;
+  return p;
+}
+
+// Turns `node` into an inner node over a new record pair whose records wait (axis =
+// kPending) to be built over [b, split) and [split, e). False if the record array is full.
+__device__ bool make_inner(const DevUpdate& u, int node, const Partition& p, int b, int e) {
+  const unsigned spot = atomicAdd(u.next_free, 2u);
+  if (spot + 1 >= (unsigned)u.n_records) {
+    atomicExch(u.err, 1);
+    return false;
+  }
+  RefNode& nd = u.refs[node];
+  nd.child = (int32_t)spot;
+  nd.children = 0;
+  nd.axis = (uint8_t)p.axis;
+  RefNode& c0 = u.refs[spot];
+  RefNode& c1 = u.refs[spot + 1];
+  set_box(c0, p.l);
+  set_box(c1, p.r);
+  c0.child = b;
+  c0.children = (int16_t)(p.split - b);
+  c0.axis = kPending;
+  c1.child = p.split;
+  c1.children = (int16_t)(e - p.split);
+  c1.axis = kPending;
+  u.parent[spot] = node;
+  u.parent[spot + 1] = node;
+  return true;
+}
+
+// DynBVH::build(leaf, begin, end, useApproximateBuild = true, nodeAlreadyExists = true):
+// depth-first over the new subtree without a stack, using parent links.
+__device__ void split_leaf(const DevUpdate& u, int root) {
+  RefNode* refs = u.refs;
+  const int b0 = refs[root].child, e0 = b0 + refs[root].children;
+  const Partition p0 = partition_approx(u, root, b0, e0);
+  if (p0.axis == -1) return;  // the leaf stays as it is
+  if (!make_inner(u, root, p0, b0, e0)) return;
+  int cur = refs[root].child;
+  for (;;) {
+    // cur waits to be built over [child, child + children)
+    const int b = refs[cur].child, e = b + refs[cur].children;
+    const Partition p = partition_approx(u, cur, b, e);
+    bool descended = false;
+    if (p.axis == -1) {
+      RefNode& nd = refs[cur];
+      nd.child = b;
+      nd.children = (int16_t)(e - b);
+      nd.axis = 0;
+      u.costs[cur] = nd.children * kCIsec;
+      u.ssize[cur] = 1;
+      for (int i = b + 1; i < e; ++i) {  // std::sort of the leaf's object ids
+        const int32_t v = u.ids[i];
+        int j = i - 1;
+        while (j >= b && u.ids[j] > v) {
+          u.ids[j + 1] = u.ids[j];
+          --j;
+        }
+        u.ids[j + 1] = v;
+      }
+    } else if (make_inner(u, cur, p, b, e)) {
+      cur = refs[cur].child;
+      descended = true;
+    } else {
+      return;  // record array full: reported through err
+    }
+    if (descended) continue;
+    // climb: finish parents whose right subtree is done, move to the next right sibling
+    for (;;) {
+      const int par = u.parent[cur];
+      if (cur == refs[par].child) {
+        cur = cur + 1;
+        break;
+      }
+      u.ssize[par] = 1 + u.ssize[refs[par].child] + u.ssize[refs[par].child + 1];
+      if (par == root) return;
+      cur = par;
+    }
+  }
+}
+
+__device__ __forceinline__ void swap_rec(const DevUpdate& u, int a, int b) {
+  int4* pa = reinterpret_cast<int4*>(u.refs + a);
+  int4* pb = reinterpret_cast<int4*>(u.refs + b);
+  const int4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+  pa[0] = b0;
+  pa[1] = b1;
+  pb[0] = a0;
+  pb[1] = a1;
+  const float c = u.costs[a];
+  u.costs[a] = u.costs[b];
+  u.costs[b] = c;
+  const uint32_t z = u.ssize[a];
+  u.ssize[a] = u.ssize[b];
+  u.ssize[b] = z;
+}
+
+__device__ __forceinline__ Box unite(const DevUpdate& u, int a, int b) {
+  Box x = rbox(u.refs[a]);
+  extend(x, rbox(u.refs[b]));
+  return x;
+}
+
+// Step 3 of rotateNode: the axis of largest overlap and child order (DynBVH.cc:3305-3352).
+__device__ void guess_axis(const DevUpdate& u, int parent, int a, int b) {
+  const Box x = rbox(u.refs[a]), y = rbox(u.refs[b]);
+  float o[3];
+  for (int k = 0; k < 3; ++k) o[k] = mx(x.lo[k], y.lo[k]) - mn(x.hi[k], y.hi[k]);
+  const int ax = index_of_max(o[0], o[1], o[2]);
+  u.refs[parent].axis = (uint8_t)ax;
+  const float ha = ax == 0 ? x.hi[0] : (ax == 1 ? x.hi[1] : x.hi[2]);
+  const float hb = ax == 0 ? y.hi[0] : (ax == 1 ? y.hi[1] : y.hi[2]);
+  if (ha > hb) swap_rec(u, a, b);
+}
+
+// DynBVH::rotateNode (DynBVH.cc:3051-3372)
+__device__ void rotate_node(const DevUpdate& u, int node) {
+  RefNode* refs = u.refs;
+  float* costs = u.costs;
+  uint32_t* ss = u.ssize;
+  const int L = refs[node].child, R = L + 1;
+  if (refs[L].children > 1) split_leaf(u, L);
+  if (refs[R].children > 1) split_leaf(u, R);
+  const int LL = refs[L].child, LR = LL + 1, RL = refs[R].child, RR = RL + 1;
+  const bool l_inner = refs[L].children == 0, r_inner = refs[R].children == 0;
+  const float nodeArea = area(rbox(refs[node]));
+  const float leftArea = area(rbox(refs[L]));
+  const float rightArea = area(rbox(refs[R]));
+  float llA = 0.f, lrA = 0.f, rlA = 0.f, rrA = 0.f;
+  Box boxA, boxB;
+  reset(boxA);
+  reset(boxB);
+  float areaA = 0.f, areaB = 0.f;
+  int best = 0;
+  float bestCost = nodeArea * kCTrav + leftArea * costs[L] + rightArea * costs[R];
+  if (l_inner) {
+    llA = area(rbox(refs[LL]));
+    lrA = area(rbox(refs[LR]));
+    const float partial = nodeArea * kCTrav + rightArea * costs[R] + lrA * costs[LR] + llA * costs[LL];
+    const Box b1 = unite(u, R, LR);
+    const float a1 = area(b1);
+    const float c1 = a1 * kCTrav + partial;
+    if (c1 < bestCost) {
+      best = 1;
+      bestCost = c1;
+      boxA = b1;
+      areaA = a1;
+    }
+    const Box b2 = unite(u, R, LL);
+    const float a2 = area(b2);
+    const float c2 = a2 * kCTrav + partial;
+    if (c2 < bestCost) {
+      best = 2;
+      bestCost = c2;
+      boxA = b2;
+      areaA = a2;
+    }
+  }
+  if (r_inner) {
+    rlA = area(rbox(refs[RL]));
+    rrA = area(rbox(refs[RR]));
+    const float partial = nodeArea * kCTrav + leftArea * costs[L] + rrA * costs[RR] + rlA * costs[RL];
+    const Box b3 = unite(u, L, RR);
+    const float a3 = area(b3);
+    const float c3 = a3 * kCTrav + partial;
+    if (c3 < bestCost) {
+      best = 3;
+      bestCost = c3;
+      boxA = b3;
+      areaA = a3;
+    }
+    const Box b4 = unite(u, L, RL);
+    const float a4 = area(b4);
+    const float c4 = a4 * kCTrav + partial;
+    if (c4 < bestCost) {
+      best = 4;
+      bestCost = c4;
+      boxA = b4;
+      areaA = a4;
+    }
+    if (l_inner) {
+      const float partial2 = nodeArea * kCTrav + llA * costs[LL] + lrA * costs[LR] + rlA * costs[RL] + rrA * costs[RR];
+      const Box b5l = unite(u, LL, RL), b5r = unite(u, LR, RR);
+      const float a5l = area(b5l), a5r = area(b5r);
+      const float c5 = (a5l + a5r) * kCTrav + partial2;
+      if (c5 < bestCost) {
+        best = 5;
+        bestCost = c5;
+        boxA = b5l;
+        areaA = a5l;
+        boxB = b5r;
+        areaB = a5r;
+      }
+      const Box b6l = unite(u, LL, RR), b6r = unite(u, RL, LR);
+      const float a6l = area(b6l), a6r = area(b6r);
+      const float c6 = (a6l + a6r) * kCTrav + partial2;
+      if (c6 < bestCost) {
+        best = 6;
+        bestCost = c6;
+        boxA = b6l;
+        areaA = a6l;
+        boxB = b6r;
+        areaB = a6r;
+      }
+    }
+  }
+  costs[node] = bestCost / nodeArea;
+  if (best == 0) return;
+  const float invA = 1 / areaA;
+  switch (best) {
+    case 1:
+      set_box(refs[L], boxA);
+      costs[L] = kCTrav + ((lrA * costs[LR] + rightArea * costs[R]) * invA);
+      swap_rec(u, LL, R);
+      ss[L] = ss[LL] + ss[LR] + 1;
+      break;
+    case 2:
+      set_box(refs[L], boxA);
+      costs[L] = kCTrav + ((llA * costs[LL] + rightArea * costs[R]) * invA);
+      swap_rec(u, LR, R);
+      ss[L] = ss[LL] + ss[LR] + 1;
+      break;
+    case 3:
+      set_box(refs[R], boxA);
+      costs[R] = kCTrav + ((rrA * costs[RR] + leftArea * costs[L]) * invA);
+      swap_rec(u, RL, L);
+      ss[R] = ss[RL] + ss[RR] + 1;
+      break;
+    case 4:
+      set_box(refs[R], boxA);
+      costs[R] = kCTrav + ((rlA * costs[RL] + leftArea * costs[L]) * invA);
+      swap_rec(u, RR, L);
+      ss[R] = ss[RL] + ss[RR] + 1;
+      break;
+    case 5:
+      set_box(refs[L], boxA);
+      set_box(refs[R], boxB);
+      costs[L] = kCTrav + ((llA * costs[LL] + rlA * costs[RL]) * invA);
+      costs[R] = kCTrav + ((lrA * costs[LR] + rrA * costs[RR]) / areaB);
+      swap_rec(u, LR, RL);
+      ss[L] = ss[LL] + ss[LR] + 1;
+      ss[R] = ss[RL] + ss[RR] + 1;
+      break;
+    default:  // 6
+      set_box(refs[L], boxA);
+      set_box(refs[R], boxB);
+      costs[L] = kCTrav + ((llA * costs[LL] + rrA * costs[RR]) * invA);
+      costs[R] = kCTrav + ((lrA * costs[LR] + rlA * costs[RL]) / areaB);
+      swap_rec(u, LR, RR);
+      ss[L] = ss[LL] + ss[LR] + 1;
+      ss[R] = ss[RL] + ss[RR] + 1;
+      break;
+  }
+  if (best <= 2 || best >= 5) guess_axis(u, L, LL, LR);
+  if (best >= 3) guess_axis(u, R, RL, RR);
+  guess_axis(u, node, L, R);
+  if (refs[node].bits & 2) {
+    auto mark = [&](int r) {
+      if (ss[r] > u.large)
+        refs[r].bits |= 2;
+      else
+        refs[r].bits &= ~2;
+    };
+    mark(L);
+    mark(R);
+    if (best <= 2 || best >= 5) {
+      mark(LL);
+      mark(LR);
+    }
+    if (best >= 3) {
+      mark(RL);
+      mark(RR);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_obj_bounds(DevUpdate u) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= u.n_objects) return;
+  Box b;
+  const float* p0 = u.verts + 3 * (size_t)u.idx[3 * i + 0];
+  for (int k = 0; k < 3; ++k) b.lo[k] = b.hi[k] = p0[k];  // Mesh::getBBox (Mesh.cc:228-237)
+  extend_pt(b, u.verts + 3 * (size_t)u.idx[3 * i + 1]);
+  extend_pt(b, u.verts + 3 * (size_t)u.idx[3 * i + 2]);
+  float* o = u.obox + 6 * (size_t)i;
+  for (int k = 0; k < 3; ++k) {
+    o[k] = b.lo[k];
+    o[3 + k] = b.hi[k];
+    // BBox::center = Interpolate(min, max, 0.5f) (BBox.h:63-65, MiscMath.h:136-139)
+    u.ocen[3 * (size_t)i + k] = b.hi[k] * 0.5f + b.lo[k] * (1 - 0.5f);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, const int32_t* leaves,
+                                                       int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  RefNode& nd = u.refs[leaves[i]];
+  Box b;
+  reset(b);
+  for (int k = 0; k < nd.children; ++k) extend(b, obj_box(u, u.ids[nd.child + k]));
+  set_box(nd, b);
+  nd.axis = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_upd_level(DevUpdate u, const int32_t* recs,
+                                                      int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int node = recs[i];
+  const int c = u.refs[node].child;
+  Box b;
+  reset(b);
+  extend(b, rbox(u.refs[c]));
+  extend(b, rbox(u.refs[c + 1]));
+  set_box(u.refs[node], b);
+  rotate_node(u, node);
+}
+
+__device__ __forceinline__ int32_t encode(const DevUpdate& u, int rec) {
+  const RefNode& n = u.refs[rec];
+  if (n.children == 0) return (n.child - 1) >> 1;
+  const int c = n.children < kLeafCountInline ? n.children : kLeafCountInline;
+  if (n.children >= kLeafCountInline) u.big_count[n.child] = n.children;
+  return ~((n.child << kLeafCountBits) | c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= (int64_t)*u.next_free) return;
+  const RefNode nd = u.refs[r];
+  if (nd.children != 0) return;
+  PairNode& P = u.pairs[(nd.child - 1) >> 1];
+  for (int side = 0; side < 2; ++side) {
+    const RefNode& c = u.refs[nd.child + side];
+    float* b = P.b + 6 * side;
+    b[0] = c.box.lo[0];
+    b[1] = c.box.hi[0];
+    b[2] = c.box.lo[1];
+    b[3] = c.box.hi[1];
+    b[4] = c.box.lo[2];
+    b[5] = c.box.hi[2];
+    P.ref[side] = encode(u, nd.child + side);
+  }
+  P.meta = (int32_t)nd.axis | ((nd.bits & 1) << 2);
+  P.node = (int32_t)r;
+}
+
+// anc = parent record (root: -1), dep = 1 (root: 0) for every record in use.
+__global__ __launch_bounds__(kBlock) void k_parents(DevUpdate u, int32_t* anc, int32_t* dep) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= (int64_t)*u.next_free) return;
+  if (r == 0) {
+    anc[0] = -1;
+    dep[0] = 0;
+  }
+  const RefNode nd = u.refs[r];
+  if (nd.children != 0) return;
+  anc[nd.child] = (int32_t)r;
+  anc[nd.child + 1] = (int32_t)r;
+  dep[nd.child] = 1;
+  dep[nd.child + 1] = 1;
+}
+
+// One pointer-jumping step: depth sums along ancestor links that double in length.
+__global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, const int32_t* anc_in,
+                                                 const int32_t* dep_in, int32_t* anc_out,
+                                                 int32_t* dep_out) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= (int64_t)*next_free) return;
+  const int32_t a = anc_in[r];
+  if (a < 0) {
+    anc_out[r] = a;
+    dep_out[r] = dep_in[r];
+    return;
+  }
+  anc_out[r] = anc_in[a];
+  dep_out[r] = dep_in[r] + dep_in[a];
+}
+
+// hist[d] = inner records at depth d (d < kMaxDepth), hist[kMaxDepth] = leaves,
+// hist[kMaxDepth + 1] = deepest depth.
+__global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* dep,
+                                                       unsigned* hist) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= (int64_t)*u.next_free) return;
+  const int d = dep[r];
+  atomicMax(hist + kMaxDepth + 1, (unsigned)d);
+  if (u.refs[r].children != 0)
+    atomicAdd(hist + kMaxDepth, 1u);
+  else if (d < kMaxDepth)
+    atomicAdd(hist + d, 1u);
+}
+
+// cursor[d]: next position for depth d in levels; cursor[kMaxDepth]: next leaf position.
+__global__ __launch_bounds__(kBlock) void k_depth_scatter(DevUpdate u, const int32_t* dep,
+                                                          unsigned* cursor, int32_t* levels,
+                                                          int32_t* leaves) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= (int64_t)*u.next_free) return;
+  if (u.refs[r].children != 0) {
+    leaves[atomicAdd(cursor + kMaxDepth, 1u)] = (int32_t)r;
+  } else {
+    levels[atomicAdd(cursor + dep[r], 1u)] = (int32_t)r;
+  }
+}
+
+unsigned grid(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+int launch_obj_bounds(const DevUpdate& u, void* stream) {
+  if (u.n_objects <= 0) return 0;
+  hipLaunchKernelGGL(k_obj_bounds, dim3(grid(u.n_objects)), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), u);
+  return (int)hipGetLastError();
+}
+
+int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
+                       const int32_t* levels, const int64_t* level_off, int n_levels,
+                       void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (n_leaves > 0)
+    hipLaunchKernelGGL(k_upd_leaves, dim3(grid(n_leaves)), dim3(kBlock), 0, s, u, leaves,
+                       n_leaves);
+  for (int d = n_levels - 1; d >= 0; --d) {
+    const int64_t first = level_off[d], n = level_off[d + 1] - level_off[d];
+    if (n > 0)
+      hipLaunchKernelGGL(k_upd_level, dim3(grid(n)), dim3(kBlock), 0, s, u, levels + first, n);
+  }
+  hipLaunchKernelGGL(k_rebuild_pairs, dim3(grid(u.n_records)), dim3(kBlock), 0, s, u);
+  return (int)hipGetLastError();
+}
+
+int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc1, int32_t* dep1,
+                  unsigned* hist, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const unsigned g = grid(u.n_records);
+  hipLaunchKernelGGL(k_parents, dim3(g), dim3(kBlock), 0, s, u, anc0, dep0);
+  // 2^8 > kMaxDepth: eight doublings reach the root from any record
+  for (int it = 0; it < 8; ++it) {
+    const bool even = (it & 1) == 0;
+    hipLaunchKernelGGL(k_jump, dim3(g), dim3(kBlock), 0, s, u.next_free, even ? anc0 : anc1,
+                       even ? dep0 : dep1, even ? anc1 : anc0, even ? dep1 : dep0);
+  }
+  hipLaunchKernelGGL(k_depth_hist, dim3(g), dim3(kBlock), 0, s, u, dep0, hist);
+  return (int)hipGetLastError();
+}
+
+int launch_depth_scatter(const DevUpdate& u, const int32_t* dep, unsigned* cursor,
+                         int32_t* levels, int32_t* leaves, void* stream) {
+  hipLaunchKernelGGL(k_depth_scatter, dim3(grid(u.n_records)), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), u, dep, cursor, levels, leaves);
+  return (int)hipGetLastError();
+}
+
+}  // namespace mh

@@ -316,6 +316,14 @@ int mh_server::round() {
     t->total = total[k];
     const size_t bytes = offsetof(mh::BatchTable, batches) + sizeof(mh::BatchDev) * nb[k];
     HIP_OK(hipMemcpyAsync(per[k]->table.p, t, bytes, hipMemcpyHostToDevice, per[k]->stream));
+    // A rotating update between rounds can deepen the tree: grow the spill area first.
+    const int64_t need = std::max<int64_t>(
+        0, (int64_t)scenes[k]->stack_bound(launch.layout) - launch.stack_lds);
+    const size_t spill_bytes = (size_t)need * per[k]->spill_stride * 8;
+    if (spill_bytes > per[k]->spill.bytes) {
+      HIP_OK(hipStreamSynchronize(per[k]->stream));
+      if (int rc = per[k]->spill.reserve(spill_bytes)) return rc;
+    }
     mh::BatchLaunch bl{};
     bl.table = per[k]->table.as<const mh::BatchTable>();
     bl.nb = nb[k];

@@ -1,8 +1,8 @@
 // Traversal kernels for gfx950 (wave64).
 //
 // k_trace_ray: one ray per lane, DynBVH::intersect on a one-ray packet
-//   (Model/Groups/DynBVH.cc:101-404) over 64-byte PairNodes or 128-byte QuadNodes with an
-//   LDS ring stack; the traversal itself is traverse.h's, shared with the batch kernel.
+//   (Model/Groups/DynBVH.cc:101-404) over 64-byte PairNodes with an LDS ring stack; the
+//   traversal itself is traverse.h's, shared with the batch kernel.
 //
 // k_trace_packet: one RayPacket of up to 64 consecutive rays per wave, the reference's
 //   packet recursion (intersectNode/firstIntersects/lastIntersects, DynBVH.cc:274-905,
@@ -270,10 +270,8 @@ int launch_any_prim(const DevScene& sc, const DevRays& rays, const DevHits& hits
                          dim3(kBlock), 0, s, sc, rays, hits, n, cfg.packet_size, c);
     return (int)hipGetLastError();
   }
-  // The statistics variant counts the binary traversal's tests (PAIR layout).
-  if (stats) return launch_ray_s<ANY, PRIM, kLayoutPair, true>(sc, rays, hits, n, cfg, s);
-  return cfg.layout == kLayoutQuad ? launch_ray_s<ANY, PRIM, kLayoutQuad, false>(sc, rays, hits, n, cfg, s)
-                                   : launch_ray_s<ANY, PRIM, kLayoutPair, false>(sc, rays, hits, n, cfg, s);
+  return stats ? launch_ray_s<ANY, PRIM, kLayoutPair, true>(sc, rays, hits, n, cfg, s)
+               : launch_ray_s<ANY, PRIM, kLayoutPair, false>(sc, rays, hits, n, cfg, s);
 }
 
 }  // namespace

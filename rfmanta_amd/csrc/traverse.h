@@ -9,16 +9,6 @@
 //   that child with the updated minT.
 // REF layout: the reference's 32-byte records; each node's own box is tested when it is
 //   visited, as firstIntersects does.
-// QUAD layout: 128-byte QuadNodes hold the boxes of a node's grandchildren (two binary
-//   levels per fetch). The four slots are visited in the binary order (the node's front
-//   child first, and inside each child its front child first), the first hit slot is
-//   entered and the other hit slots are pushed with their entry distances, to be entered
-//   only if that distance is still <= minT when popped. This gives the binary traversal's
-//   results exactly: a child box lies inside its parent box, and the slab test is
-//   monotone in the plane coordinates, so a grandchild's interval lies inside its
-//   parent's; a grandchild tested at the grandparent's visit and checked against minT
-//   at its pop is therefore entered exactly when the binary traversal would have entered
-//   its parent and then it, and the leaves are reached in the same order.
 // The stack is a per-lane ring of S entries in LDS ([slot][lane], conflict free) whose
 // oldest entries spill to a per-thread global area when the ring is full.
 #pragma once
@@ -152,104 +142,6 @@ __device__ __forceinline__ void trace_pair(const DevScene& sc, const RefNode& ro
   }
 }
 
-// v[k] for k in 0..3, chosen with bit masks (no indexed, scratch-backed access).
-__device__ __forceinline__ unsigned pick4u(int k, unsigned a, unsigned b, unsigned c, unsigned d) {
-  const unsigned m0 = 0u - (unsigned)(k == 0), m1 = 0u - (unsigned)(k == 1),
-                 m2 = 0u - (unsigned)(k == 2), m3 = 0u - (unsigned)(k == 3);
-  return (a & m0) | (b & m1) | (c & m2) | (d & m3);
-}
-
-template <bool ANY, int PRIM, int S, int BLOCK, bool STATS, bool FAST>
-__device__ __forceinline__ void trace_quad(const DevScene& sc, const RefNode& root, const Ray& r,
-                                           RayHit& h, LaneStack<S, BLOCK>& st,
-                                           unsigned long long& nbox, unsigned long long& ntri) {
-  float tn;
-  bool go = slab_v<FAST>(root.box.lo[0], root.box.hi[0], root.box.lo[1], root.box.hi[1],
-                         root.box.lo[2], root.box.hi[2], r, h.minT, tn);
-  if (STATS) ++nbox;
-  int cur = sc.quad_root;
-  st.reset();
-  while (go) {
-    if (cur >= 0) {
-      const float4* np = reinterpret_cast<const float4*>(sc.quads + cur);
-      const float4 B0 = np[0], B1 = np[1], B2 = np[2], B3 = np[3], B4 = np[4], B5 = np[5];
-      const int4 R = reinterpret_cast<const int4*>(np)[6];
-      const int meta = reinterpret_cast<const int*>(np)[28];
-      float t0, t1, t2, t3;
-      const bool h0 = ((meta >> 9) & 1) && slab_v<FAST>(B0.x, B0.y, B0.z, B0.w, B1.x, B1.y, r, h.minT, t0);
-      const bool h1 = ((meta >> 10) & 1) && slab_v<FAST>(B1.z, B1.w, B2.x, B2.y, B2.z, B2.w, r, h.minT, t1);
-      const bool h2 = ((meta >> 11) & 1) && slab_v<FAST>(B3.x, B3.y, B3.z, B3.w, B4.x, B4.y, r, h.minT, t2);
-      const bool h3 = ((meta >> 12) & 1) && slab_v<FAST>(B4.z, B4.w, B5.x, B5.y, B5.z, B5.w, r, h.minT, t3);
-      // Visit order: front child of N, then inside each child its front child.
-      bool n_first, a_first, b_first;  // true: child + 0 first
-      if (ANY) {
-        n_first = (meta >> 2) & 1;
-        a_first = (meta >> 5) & 1;
-        b_first = (meta >> 8) & 1;
-      } else {
-        n_first = (r.dpos >> (meta & 3)) & 1;
-        a_first = (r.dpos >> ((meta >> 3) & 3)) & 1;
-        b_first = (r.dpos >> ((meta >> 6) & 3)) & 1;
-      }
-      const int sa0 = a_first ? 0 : 1, sb0 = b_first ? 2 : 3;
-      const int o0 = n_first ? sa0 : sb0, o1 = n_first ? 1 - sa0 : 5 - sb0;
-      const int o2 = n_first ? sb0 : sa0, o3 = n_first ? 5 - sb0 : 1 - sa0;
-      const unsigned hm = (unsigned)h0 | ((unsigned)h1 << 1) | ((unsigned)h2 << 2) |
-                          ((unsigned)h3 << 3);
-      const bool g0 = (hm >> o0) & 1, g1 = (hm >> o1) & 1, g2 = (hm >> o2) & 1,
-                 g3 = (hm >> o3) & 1;
-      const unsigned ut0 = __float_as_uint(t0), ut1 = __float_as_uint(t1),
-                     ut2 = __float_as_uint(t2), ut3 = __float_as_uint(t3);
-      const unsigned ur0 = (unsigned)R.x, ur1 = (unsigned)R.y, ur2 = (unsigned)R.z,
-                     ur3 = (unsigned)R.w;
-#define MH_QREF(o) ((int)pick4u(o, ur0, ur1, ur2, ur3))
-#define MH_QTN(o) __uint_as_float(pick4u(o, ut0, ut1, ut2, ut3))
-      if (STATS) nbox += 4;
-      if (g0) {
-        if (g3) st.push(MH_QREF(o3), MH_QTN(o3));
-        if (g2) st.push(MH_QREF(o2), MH_QTN(o2));
-        if (g1) st.push(MH_QREF(o1), MH_QTN(o1));
-        cur = MH_QREF(o0);
-        continue;
-      }
-      if (g1) {
-        if (g3) st.push(MH_QREF(o3), MH_QTN(o3));
-        if (g2) st.push(MH_QREF(o2), MH_QTN(o2));
-        cur = MH_QREF(o1);
-        continue;
-      }
-      if (g2) {
-        if (g3) st.push(MH_QREF(o3), MH_QTN(o3));
-        cur = MH_QREF(o2);
-        continue;
-      }
-      if (g3) {
-        cur = MH_QREF(o3);
-        continue;
-      }
-#undef MH_QREF
-#undef MH_QTN
-    } else {
-      const int code = ~cur;
-      const int first = code >> kLeafCountBits;
-      int cnt = code & kLeafCountInline;
-      if (cnt == kLeafCountInline) cnt = sc.big_count[first];
-      if (leaf<ANY, PRIM, STATS>(sc, first, cnt, r, h, ntri)) return;
-    }
-    go = false;
-    while (!st.empty()) {
-      int ref;
-      float t;
-      st.pop(ref, t);
-      if (t <= h.minT) {
-        cur = ref;
-        go = true;
-        break;
-      }
-    }
-  }
-}
-
 template <bool ANY, int PRIM, int S, int BLOCK, bool STATS, bool FAST>
 __device__ __forceinline__ void trace_ref(const DevScene& sc, const Ray& r, RayHit& h,
                                           LaneStack<S, BLOCK>& st, unsigned long long& nbox,
@@ -295,15 +187,11 @@ __device__ __forceinline__ void trace_store(const DevScene& sc, const RefNode& r
   if (__ballot(!regular_ray(r, h.minT)) == 0ull) {
     if (LAYOUT == kLayoutPair)
       trace_pair<ANY, PRIM, S, BLOCK, STATS, true>(sc, root, r, h, st, nbox, ntri);
-    else if (LAYOUT == kLayoutQuad)
-      trace_quad<ANY, PRIM, S, BLOCK, STATS, true>(sc, root, r, h, st, nbox, ntri);
     else
       trace_ref<ANY, PRIM, S, BLOCK, STATS, true>(sc, r, h, st, nbox, ntri);
   } else {
     if (LAYOUT == kLayoutPair)
       trace_pair<ANY, PRIM, S, BLOCK, STATS, false>(sc, root, r, h, st, nbox, ntri);
-    else if (LAYOUT == kLayoutQuad)
-      trace_quad<ANY, PRIM, S, BLOCK, STATS, false>(sc, root, r, h, st, nbox, ntri);
     else
       trace_ref<ANY, PRIM, S, BLOCK, STATS, false>(sc, r, h, st, nbox, ntri);
   }

@@ -209,11 +209,11 @@ void HipDynBVH::rebuildSerial() {
     if (needToSaveFile) saveToFile(saveFileName);
     return;
   }
-  // Same group, moved vertices: refit on the device (DynBVH::update).
+  // Same group, moved vertices: DynBVH::update on the device, i.e. the bounds refit with
+  // tree rotations and leaf splits (DynBVH.cc:1574-1581, 2012-2036 with numProcs == 1).
   const float* verts = &mesh_->vertices[0].data[0];
   if (mh_scene_set_vertices(scene_, verts, 0, 0) != MH_OK) fail("update");
-  if (mh_scene_refit(scene_, 0) != MH_OK) fail("update");
-  if (mh_stream_sync(0) != MH_OK) fail("update");
+  if (mh_scene_update(scene_, 0) != MH_OK) fail("update");
 }
 
 // DynBVH::buildFromFile (DynBVH.cc:2442-2472).

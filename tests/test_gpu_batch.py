@@ -55,11 +55,6 @@ def _device(batches):
     {"layout": 0, "stack_lds": 16, "block": 64, "blocks_per_cu": 8, "refill": 64, "chunk": 0,
      "xcd_remap": 1},
     {"layout": 1, "stack_lds": 8, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 0},
-    # QUAD layout (whole-wave schedules), small rings so that the stack spills
-    {"layout": 2, "stack_lds": 8, "block": 256, "blocks_per_cu": 4, "refill": 64, "chunk": 0},
-    {"layout": 2, "stack_lds": 8, "block": 64, "blocks_per_cu": 6, "refill": 64, "chunk": 128,
-     "xcd_remap": 1},
-    {"layout": 2, "stack_lds": 32, "block": 128, "blocks_per_cu": 3, "refill": 64, "chunk": 64},
 ])
 def test_frame_matches_oracle(oracle, small, cfg):
     v, i, batches = small

@@ -58,14 +58,11 @@ def test_built_tree_matches_oracle(c0_scene, c0_tree):
     assert nodes.tobytes() == o_nodes.tobytes()
 
 
-LAYOUTS = [(1, L.MH_LAYOUT_PAIR), (1, L.MH_LAYOUT_QUAD), (64, L.MH_LAYOUT_PAIR)]
-
-
-@pytest.mark.parametrize("packet,layout", LAYOUTS)
-def test_c0_primary_closest_hit_exact(oracle, c0, c0_tree, c0_scene, packet, layout):
+@pytest.mark.parametrize("packet", [1, 64])
+def test_c0_primary_closest_hit_exact(oracle, c0, c0_tree, c0_scene, packet):
     v, i, rays = c0
-    print(f"c0 primary packet={packet} layout={layout}", flush=True)
-    g = c0_scene.trace_host(rays, packet_size=packet, layout=layout)
+    print(f"c0 primary packet={packet}", flush=True)
+    g = c0_scene.trace_host(rays, packet_size=packet)
     o = oracle.trace(v, i, *c0_tree, rays, packet_size=packet)
     assert (g["prim"] >= 0).sum() > 20000
     assert np.array_equal(g["prim"], o["prim"])
@@ -74,13 +71,13 @@ def test_c0_primary_closest_hit_exact(oracle, c0, c0_tree, c0_scene, packet, lay
     assert _same_bits(g["v"], o["v"])
 
 
-@pytest.mark.parametrize("packet,layout", LAYOUTS)
-def test_c0_shadow_any_hit_exact(oracle, c0, c0_tree, c0_scene, packet, layout):
+@pytest.mark.parametrize("packet", [1, 64])
+def test_c0_shadow_any_hit_exact(oracle, c0, c0_tree, c0_scene, packet):
     v, i, rays = c0
-    print(f"c0 shadow packet={packet} layout={layout}", flush=True)
+    print(f"c0 shadow packet={packet}", flush=True)
     prim = oracle.trace(v, i, *c0_tree, rays, packet_size=1)
     sh = scenes.shadow_rays(v, i, rays, prim, scenes.CONFIG0_LIGHT, compact=False)
-    g = c0_scene.trace_host(sh, any_hit=True, packet_size=packet, layout=layout)
+    g = c0_scene.trace_host(sh, any_hit=True, packet_size=packet)
     o = oracle.trace(v, i, *c0_tree, sh, any_hit=True, packet_size=packet)
     assert np.array_equal(g["occluded"], o["occluded"])
     assert np.array_equal(g["prim"], o["prim"])  # same visit order: same first occluder
@@ -107,8 +104,8 @@ def test_wald_exact(oracle, c0, c0_tree):
     v, i, rays = c0
     with api.Scene(v, i) as s:
         s.load_dynbvh(c0_tree[1], c0_tree[0], prim_kind=L.MH_PRIM_WALD)
-        for packet, layout in LAYOUTS:
-            g = s.trace_host(rays, packet_size=packet, layout=layout)
+        for packet in (1, 64):
+            g = s.trace_host(rays, packet_size=packet)
             o = oracle.trace(v, i, *c0_tree, rays, packet_size=packet, prim_kind=1)
             assert np.array_equal(g["prim"], o["prim"])
             assert _same_bits(g["t"], o["t"])
@@ -130,9 +127,8 @@ def test_counters_match_oracle(oracle, c0, c0_tree, c0_scene, packet):
     assert int(c[1]) == o["tri_tests"]
 
 
-@pytest.mark.parametrize("layout", [L.MH_LAYOUT_PAIR, L.MH_LAYOUT_QUAD])
 @pytest.mark.parametrize("stack", [8, 16, 32])
-def test_stack_ring_and_spill(oracle, stack, layout):
+def test_stack_ring_and_spill(oracle, stack):
     """Deep heightfield tree with small LDS rings: spills to global memory stay exact."""
     v, i = scenes.heightfield(200, 100, seed=3)
     rays = scenes.pinhole_rays((0.0, 0.9, -1.1), (0, 0, 0), (0, 1, 0), 60, 60, 128, 128)
@@ -141,7 +137,7 @@ def test_stack_ring_and_spill(oracle, stack, layout):
     with api.Scene(v, i) as s:
         s.load_dynbvh(ids, nodes)
         assert s.stats()["depth"] > 8
-        g = s.trace_host(rays, packet_size=1, stack_lds=stack, layout=layout)
+        g = s.trace_host(rays, packet_size=1, stack_lds=stack)
     assert np.array_equal(g["prim"], o["prim"])
     assert _same_bits(g["t"], o["t"])
 
@@ -162,10 +158,9 @@ def test_refit_bit_exact(oracle):
             assert gn.tobytes() == ref.tobytes(), f"frame {frame}"
             rays = scenes.pinhole_rays((0.0, 0.9, -1.1), (0, 0, 0), (0, 1, 0), 60, 60, 64, 64)
             o = oracle.trace(vd, i, ref, ids, rays, packet_size=1)
-            for layout in (L.MH_LAYOUT_PAIR, L.MH_LAYOUT_QUAD):  # both node copies refit
-                g = s.trace_host(rays, layout=layout)
-                assert np.array_equal(g["prim"], o["prim"]), (frame, layout)
-                assert _same_bits(g["t"], o["t"]), (frame, layout)
+            g = s.trace_host(rays)
+            assert np.array_equal(g["prim"], o["prim"]), frame
+            assert _same_bits(g["t"], o["t"]), frame
 
 
 def test_edge_cases(oracle):
@@ -183,11 +178,11 @@ def test_edge_cases(oracle):
     with api.Scene(v, i) as s:
         s.load_dynbvh(ids, nodes)
         for any_hit in (False, True):
-            for packet, layout in LAYOUTS + [(7, L.MH_LAYOUT_PAIR)]:
-                g = s.trace_host(rays, any_hit=any_hit, packet_size=packet, layout=layout)
+            for packet in (1, 64, 7):
+                g = s.trace_host(rays, any_hit=any_hit, packet_size=packet)
                 o = oracle.trace(v, i, nodes, ids, rays, any_hit=any_hit, packet_size=packet)
-                assert np.array_equal(g["prim"], o["prim"]), (any_hit, packet, layout)
-                assert _same_bits(g["t"], o["t"]), (any_hit, packet, layout)
+                assert np.array_equal(g["prim"], o["prim"]), (any_hit, packet)
+                assert _same_bits(g["t"], o["t"]), (any_hit, packet)
                 if any_hit:
                     assert np.array_equal(g["occluded"], o["occluded"])
     assert (g["prim"][rays["masked"] == 1] == L.MH_MASKED).all()
@@ -202,7 +197,7 @@ def test_single_triangle_and_tiny_meshes(oracle):
             s.build()
             gi, gn = s.export_dynbvh()
             assert gn.tobytes() == nodes.tobytes()
-            for packet, layout in LAYOUTS:  # T = 1: the root is a leaf
-                g = s.trace_host(rays, packet_size=packet, layout=layout)
+            for packet in (1, 64):  # T = 1: the root is a leaf
+                g = s.trace_host(rays, packet_size=packet)
                 o = oracle.trace(v, i, nodes, ids, rays, packet_size=packet)
-                assert np.array_equal(g["prim"], o["prim"]), (T, packet, layout)
+                assert np.array_equal(g["prim"], o["prim"]), (T, packet)

@@ -1,0 +1,118 @@
+"""Rotating update (mh_scene_update, DynBVH::update with tree rotations and leaf splits)
+against the oracle's restatement, frame after frame.
+
+The device takes new record pairs for split leaves from an atomic counter, so record
+numbers may differ from the oracle's serial ones; trees are therefore compared in
+canonical form: from the root down, every record's box bits, axis, flag bits and leaf
+size, each leaf's object ids, and the order of every node's two children. Hits are
+compared bit for bit against the oracle's traversal of its own tree.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from rfmanta_amd import api, scenes  # noqa: E402
+
+
+def canonical_equal(ga, gi, oa, oi):
+    """Walks both trees from the root; returns (ok, message, records)."""
+    stack = [(0, 0)]
+    n = 0
+    fields = ("lo", "hi", "axis", "bits", "children")
+    while stack:
+        g, o = stack.pop()
+        n += 1
+        for f in fields:
+            a, b = ga[f][g], oa[f][o]
+            if np.asarray(a).tobytes() != np.asarray(b).tobytes():
+                return False, f"record g{g}/o{o}: {f} {a} != {b}", n
+        c = int(ga["children"][g])
+        if c:
+            s0, s1 = int(ga["child"][g]), int(oa["child"][o])
+            if not np.array_equal(gi[s0:s0 + c], oi[s1:s1 + c]):
+                return False, f"leaf g{g}/o{o}: ids differ", n
+        else:
+            cg, co = int(ga["child"][g]), int(oa["child"][o])
+            stack.append((cg + 1, co + 1))
+            stack.append((cg, co))
+    return True, "", n
+
+
+@pytest.mark.parametrize("mesh", ["heightfield", "soup"])
+def test_update_matches_oracle(oracle, mesh):
+    if mesh == "heightfield":
+        v, i = scenes.heightfield(120, 60, seed=2)
+        amp = 0.03
+    else:
+        v, i = scenes.soup(4000, seed=4)
+        amp = 0.4
+    nodes, ids, nn = oracle.build_bvh(v, i)
+    st = oracle.RotState(nodes, ids, nn)
+    rays = scenes.random_rays(4096, 8, (-1.3, -1.3, -1.3), (1.3, 1.3, 1.3))
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes)
+        rotated = 0
+        for f in range(6):
+            print(f"update {mesh} frame {f}", flush=True)
+            vd = v.copy()
+            vd[:, 1] += (amp * np.sin(4 * v[:, 0] + 0.9 * f)).astype(np.float32)
+            before = st.nodes["child"].copy()
+            st.update(vd, i)
+            rotated += int((st.nodes["child"] != before).sum())
+            s.set_vertices(vd)
+            s.update()
+            api.sync()
+            gi, gn = s.export_dynbvh()
+            ok, msg, count = canonical_equal(gn, gi, st.nodes, st.ids)
+            assert ok, (f, msg)
+            assert count == st.next_free.value
+            stt = s.stats()
+            assert stt["nodes"] == count
+            g = s.trace_host(rays)
+            o = oracle.trace(vd, i, st.nodes, st.ids, rays, packet_size=1)
+            assert np.array_equal(g["prim"], o["prim"]), f
+            assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32)), f
+            sh = s.trace_host(rays, any_hit=True)
+            osh = oracle.trace(vd, i, st.nodes, st.ids, rays, any_hit=True, packet_size=1)
+            assert np.array_equal(sh["occluded"], osh["occluded"]), f
+        assert rotated > 0
+
+
+def test_update_then_refit_and_frames(oracle):
+    """After rotating updates the bounds-only refit, frames and packets still agree."""
+    v, i = scenes.soup(3000, seed=6)
+    nodes, ids, nn = oracle.build_bvh(v, i)
+    st = oracle.RotState(nodes, ids, nn)
+    rays = scenes.random_rays(2048, 3, (-1.3, -1.3, -1.3), (1.3, 1.3, 1.3))
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes)
+        for f in range(3):
+            vd = v.copy()
+            vd[:, 0] += (0.3 * np.sin(3 * v[:, 1] + f)).astype(np.float32)
+            st.update(vd, i)
+            s.set_vertices(vd)
+            s.update()
+        # a bounds-only refit on the rotated topology
+        vd2 = vd.copy()
+        vd2[:, 2] += np.float32(0.05)
+        s.set_vertices(vd2)
+        s.refit()
+        api.sync()
+        ref = st.nodes.copy()
+        oracle.refit(vd2, i, ref, st.ids)
+        gi, gn = s.export_dynbvh()
+        ok, msg, _ = canonical_equal(gn, gi, ref, st.ids)
+        assert ok, msg
+        o = oracle.trace(vd2, i, ref, st.ids, rays, packet_size=64)
+        g = s.trace_host(rays, packet_size=64)
+        assert np.array_equal(g["prim"], o["prim"])
+        dr = api.DeviceRays.upload(rays)
+        dh = api.DeviceHits.alloc(dr.n)
+        fr = api.Frame(s, [(dr, dh, False)])
+        fr.launch()
+        api.sync()
+        o1 = oracle.trace(vd2, i, ref, st.ids, rays, packet_size=1)
+        assert np.array_equal(dh.download()["prim"], o1["prim"])
+        fr.close()
