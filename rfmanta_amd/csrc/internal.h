@@ -80,6 +80,9 @@ struct mh_scene {
   int prim_kind = MH_PRIM_KENSLER_SHIRLEY;
   int root_ref = 0;
   uint32_t next_free = 0;  // records in use (the reference's nextFree)
+  // The rotating update needs records [0, next_free) to be exactly the tree and the
+  // leaves to partition the object slots; trees loaded from elsewhere may not be.
+  bool updatable = false;
   uint32_t n_inner = 0, n_leaves = 0, depth = 0;
 
   DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, leaves, levels;

@@ -133,11 +133,16 @@ int upload(mh_scene* s) {
   };
   std::vector<It> st;
   st.push_back({0, 0});
-  uint32_t max_depth = 0, n_inner = 0, max_rec = 0;
+  uint32_t max_depth = 0, n_inner = 0, max_rec = 0, n_reached = 0;
+  uint64_t leaf_objects = 0;
+  std::vector<uint8_t> seen(nrec, 0);
   while (!st.empty()) {
     It it = st.back();
     st.pop_back();
     const mh::RefNode& n = nodes[it.rec];
+    if (seen[it.rec]) return fail(MH_E_INVALID, "record %d is reached twice: not a tree", it.rec);
+    seen[it.rec] = 1;
+    ++n_reached;
     max_depth = std::max(max_depth, it.depth);
     max_rec = std::max(max_rec, (uint32_t)it.rec);
     if (n.leaf()) {
@@ -145,6 +150,7 @@ int upload(mh_scene* s) {
       if ((uint32_t)n.child + (uint32_t)n.children > T || n.child < 0)
         return fail(MH_E_INVALID, "record %d: leaf range outside object_ids", it.rec);
       leaves.push_back(it.rec);
+      leaf_objects += (uint32_t)n.children;
       if (n.children >= mh::kLeafCountInline) big[n.child] = n.children;
       continue;
     }
@@ -193,6 +199,16 @@ int upload(mh_scene* s) {
   s->n_leaves = (uint32_t)leaves.size();
   s->depth = max_depth;
   s->next_free = max_rec + 1;
+  // Leaf ranges lie inside [0, T); they partition it iff their sizes sum to T and no slot
+  // is covered twice.
+  bool partition = leaf_objects == T;
+  if (partition) {
+    std::vector<uint8_t> cover(T, 0);
+    for (int32_t r : leaves)
+      for (int k = 0; k < nodes[r].children && partition; ++k)
+        partition = !cover[nodes[r].child + k]++;
+  }
+  s->updatable = partition && n_reached == max_rec + 1;
   std::vector<int32_t> levels;
   s->level_off.assign(1, 0);
   for (auto& v : by_depth) {
@@ -579,6 +595,9 @@ int mh_scene_refit(mh_scene* s, void* stream) {
 int mh_scene_update(mh_scene* s, void* stream) {
   if (!s) return fail(MH_E_INVALID, "null scene");
   if (!s->built) return fail(MH_E_STATE, "scene has no tree");
+  if (!s->updatable)
+    return fail(MH_E_UNSUPPORTED, "the tree has records outside it or leaves that do not "
+                "partition the objects; use mh_scene_refit");
   int rc = ensure_init();
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -634,10 +653,17 @@ int mh_scene_update(mh_scene* s, void* stream) {
   HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
   HIP_OK(hipMemcpyAsync(back, nfree, 8, hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
-  if (back[1]) return fail(MH_E_STATE, "leaf splits ran out of node records");
+  // Either failure leaves a device tree that must not be traced: the scene needs a build.
+  if (back[1]) {
+    s->built = s->updatable = false;
+    return fail(MH_E_STATE, "leaf splits ran out of node records; build the scene again");
+  }
   const unsigned deepest = hist[mh::kMaxDepth + 1];
-  if (deepest >= (unsigned)mh::kMaxDepth - 1)
-    return fail(MH_E_UNSUPPORTED, "tree deeper than %d levels after rotations", mh::kMaxDepth - 1);
+  if (deepest >= (unsigned)mh::kMaxDepth - 1) {
+    s->built = s->updatable = false;
+    return fail(MH_E_UNSUPPORTED, "tree deeper than %d levels after rotations; build the "
+                "scene again", mh::kMaxDepth - 1);
+  }
   s->next_free = back[0];
   std::vector<unsigned> cursor(mh::kMaxDepth + 1, 0u);
   s->level_off.assign(1, 0);

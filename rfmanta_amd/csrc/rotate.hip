@@ -557,6 +557,13 @@ __global__ __launch_bounds__(kBlock) void k_upd_level(DevUpdate u, const int32_t
   rotate_node(u, node);
 }
 
+// Records in use: the counter, capped at the record array (a split that found the array
+// full has already raised err, and the host discards the frame).
+__device__ __forceinline__ int64_t used(const unsigned* next_free, int32_t n_records) {
+  const unsigned n = *next_free;
+  return n < (unsigned)n_records ? (int64_t)n : (int64_t)n_records;
+}
+
 __device__ __forceinline__ int32_t encode(const DevUpdate& u, int rec) {
   const RefNode& n = u.refs[rec];
   if (n.children == 0) return (n.child - 1) >> 1;
@@ -567,7 +574,7 @@ __device__ __forceinline__ int32_t encode(const DevUpdate& u, int rec) {
 
 __global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= (int64_t)*u.next_free) return;
+  if (r >= used(u.next_free, u.n_records)) return;
   const RefNode nd = u.refs[r];
   if (nd.children != 0) return;
   PairNode& P = u.pairs[(nd.child - 1) >> 1];
@@ -589,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
 // anc = parent record (root: -1), dep = 1 (root: 0) for every record in use.
 __global__ __launch_bounds__(kBlock) void k_parents(DevUpdate u, int32_t* anc, int32_t* dep) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= (int64_t)*u.next_free) return;
+  if (r >= used(u.next_free, u.n_records)) return;
   if (r == 0) {
     anc[0] = -1;
     dep[0] = 0;
@@ -603,11 +610,11 @@ __global__ __launch_bounds__(kBlock) void k_parents(DevUpdate u, int32_t* anc, i
 }
 
 // One pointer-jumping step: depth sums along ancestor links that double in length.
-__global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, const int32_t* anc_in,
-                                                 const int32_t* dep_in, int32_t* anc_out,
-                                                 int32_t* dep_out) {
+__global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, int32_t n_records,
+                                                 const int32_t* anc_in, const int32_t* dep_in,
+                                                 int32_t* anc_out, int32_t* dep_out) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= (int64_t)*next_free) return;
+  if (r >= used(next_free, n_records)) return;
   const int32_t a = anc_in[r];
   if (a < 0) {
     anc_out[r] = a;
@@ -623,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, cons
 __global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* dep,
                                                        unsigned* hist) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= (int64_t)*u.next_free) return;
+  if (r >= used(u.next_free, u.n_records)) return;
   const int d = dep[r];
   atomicMax(hist + kMaxDepth + 1, (unsigned)d);
   if (u.refs[r].children != 0)
@@ -637,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_depth_scatter(DevUpdate u, const int
                                                           unsigned* cursor, int32_t* levels,
                                                           int32_t* leaves) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= (int64_t)*u.next_free) return;
+  if (r >= used(u.next_free, u.n_records)) return;
   if (u.refs[r].children != 0) {
     leaves[atomicAdd(cursor + kMaxDepth, 1u)] = (int32_t)r;
   } else {
@@ -680,7 +687,8 @@ int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc
   // 2^8 > kMaxDepth: eight doublings reach the root from any record
   for (int it = 0; it < 8; ++it) {
     const bool even = (it & 1) == 0;
-    hipLaunchKernelGGL(k_jump, dim3(g), dim3(kBlock), 0, s, u.next_free, even ? anc0 : anc1,
+    hipLaunchKernelGGL(k_jump, dim3(g), dim3(kBlock), 0, s, u.next_free, u.n_records,
+                       even ? anc0 : anc1,
                        even ? dep0 : dep1, even ? anc1 : anc0, even ? dep1 : dep0);
   }
   hipLaunchKernelGGL(k_depth_hist, dim3(g), dim3(kBlock), 0, s, u, dep0, hist);
