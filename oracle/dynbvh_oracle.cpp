@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1063,9 +1064,10 @@ struct Updater {
     for (int i = objBegin; i < objEnd; ++i) {
       const Box& ob = obj_bounds[ids[i]];
       const float c = obj_centroids[ids[i]].v[axis];
-      // int(NaN) is INT_MIN on x86 (cvttss2si), clamped to bin 0 like the device's 0.
       const float f = (c - cb.lo[axis]) * scale[axis];
-      int which = (f != f) ? 0 : (int)f;
+      // int(f) as the reference's x86 build converts it (cvttss2si): NaN and values outside
+      // the int range give INT_MIN, i.e. bin 0 after the clamp.
+      int which = (f >= -2147483648.f && f < 2147483648.f) ? (int)f : INT_MIN;
       which = which < 0 ? 0 : (which > S - 1 ? S - 1 : which);
       counts[which]++;
       bins[which].extendBox(ob);
@@ -1374,6 +1376,89 @@ int oracle_build(const float* verts, int nverts, const uint32_t* idx, int ntris,
   std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(Node32));
   std::memcpy(ids_out, ids.data(), ids.size() * sizeof(int32_t));
   return b.num_nodes;
+}
+
+// DynBVH::rebuild(0, 1) built with USE_APPROXIMATE_BUILD set (the MANTA_USE_DYNBVH_APPROXIMATE
+// option, include/CMakeLists.txt:134-147, DynBVH.cc:1815-1822): the same rebuild
+// (DynBVH.cc:1699-1793) whose every node is partitioned by partitionApproxSAH; leaves sorted,
+// computeSubTreeSizes marks, then the RTSAH pass when rtsah != 0. Returns num_nodes.
+int oracle_build_approx(const float* verts, int nverts, const uint32_t* idx, int ntris, int rtsah,
+                        void* nodes_out, int32_t* ids_out) {
+  if (ntris <= 0) return -1;
+  Mesh m{verts, nverts, idx, ntris};
+  const size_t nrec = 2 * (size_t)ntris;
+  std::vector<Node32> nodes(nrec);
+  for (auto& nd : nodes) {
+    nd.bounds.reset();
+    nd.child = 0;
+    nd.axis = 0;
+    nd.bits = 0;
+    nd.children = 0;
+  }
+  std::vector<int32_t> ids(ntris);
+  std::vector<float> costs(nrec, 0.f);
+  std::vector<uint32_t> ssize(nrec, 0);
+  Box root;  // Mesh::computeBounds: box of all vertices (Mesh.cc:195-225)
+  root.reset();
+  for (int v = 0; v < nverts; ++v) root.extendPoint(verts + 3 * (size_t)v);
+  nodes[0].bounds = root;
+  int next_free = 1;
+  Updater u(m, nodes.data(), ids.data(), costs.data(), ssize.data(), next_free,
+            (uint32_t)(nrec / 20));
+  u.obj_bounds.resize(ntris);
+  u.obj_centroids.resize(ntris);
+  for (int i = 0; i < ntris; ++i) {
+    ids[i] = i;
+    u.obj_bounds[i] = triBox(m, i);
+    for (int a = 0; a < 3; ++a)
+      u.obj_centroids[i].v[a] = u.obj_bounds[i].hi[a] * 0.5f + u.obj_bounds[i].lo[a] * (1 - 0.5f);
+  }
+  u.buildApprox(0, 0, ntris, false);
+  // computeSubTreeSizes (DynBVH.cc:1971-1996), largeSubtreeSize = nodes.size() / 20
+  const uint32_t large = (uint32_t)(nrec / 20);
+  struct Marks {
+    std::vector<Node32>& n;
+    uint32_t large;
+    uint32_t go(int id) {
+      if (n[id].isLeaf()) {
+        n[id].bits &= ~2;
+        return 1;
+      }
+      const uint32_t s = go(n[id].child) + go(n[id].child + 1) + 1;
+      if (s > large)
+        n[id].bits |= 2;
+      else
+        n[id].bits &= ~2;
+      return s;
+    }
+  } marks{nodes, large};
+  marks.go(0);
+  if (rtsah) {
+    TTP ttp;
+    subtreeCost(nodes, ttp, 0);
+  }
+  std::memcpy(nodes_out, nodes.data(), nrec * sizeof(Node32));
+  std::memcpy(ids_out, ids.data(), ids.size() * sizeof(int32_t));
+  return next_free;  // every allocated record is in the tree
+}
+
+// partitionObjects (DynBVH.cc:1369-1396) alone: slots 0..n-1 hold objects 0..n-1 whose
+// centroid on axis 0 is keys[i]; perm receives the object order after the partition.
+// Returns the split (first slot of the right side).
+int oracle_partition_objects(const float* keys, int n, float position, int32_t* perm) {
+  Mesh m{nullptr, 0, nullptr, 0};
+  std::vector<Node32> nodes(1);
+  std::vector<float> costs(1);
+  std::vector<uint32_t> ssize(1);
+  int nf = 1;
+  Updater u(m, nodes.data(), perm, costs.data(), ssize.data(), nf, 0);
+  u.obj_centroids.resize(n);
+  for (int i = 0; i < n; ++i) {
+    perm[i] = i;
+    u.obj_centroids[i].v[0] = keys[i];
+    u.obj_centroids[i].v[1] = u.obj_centroids[i].v[2] = 0.f;
+  }
+  return u.partitionObjects(0, n, 0, position);
 }
 
 // TreeTraversalProb::computeProbs for one parent/left/right triple; boxes are

@@ -36,6 +36,10 @@ def load() -> C.CDLL:
         P = C.c_void_p
         lib.oracle_build.restype = C.c_int
         lib.oracle_build.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
+        lib.oracle_build_approx.restype = C.c_int
+        lib.oracle_build_approx.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
+        lib.oracle_partition_objects.restype = C.c_int
+        lib.oracle_partition_objects.argtypes = [P, C.c_int, C.c_float, P]
         lib.oracle_rtsah.restype = None
         lib.oracle_rtsah.argtypes = [P, C.c_int]
         lib.oracle_trace.restype = C.c_int
@@ -79,6 +83,27 @@ def build_bvh(verts, idx, rtsah: bool = True):
     if used < 0:
         raise ValueError("oracle build failed")
     return nodes, ids, used
+
+
+def build_bvh_approx(verts, idx, rtsah: bool = True):
+    """DynBVH::rebuild(0,1) built with the approximate (binned) SAH at every node, + RTSAH.
+    Returns (nodes[2T] NODE_DTYPE, ids int32[T], num_nodes)."""
+    v, i = _mesh(verts, idx)
+    T = len(i)
+    nodes = np.zeros(2 * T, NODE_DTYPE)
+    ids = np.zeros(T, np.int32)
+    used = load().oracle_build_approx(_p(v), len(v), _p(i), T, int(rtsah), _p(nodes), _p(ids))
+    if used < 0:
+        raise ValueError("oracle build failed")
+    return nodes, ids, used
+
+
+def partition_objects(keys, position):
+    """partitionObjects on objects 0..n-1 with centroids `keys`: (order after, split)."""
+    k = np.ascontiguousarray(keys, np.float32)
+    perm = np.zeros(len(k), np.int32)
+    split = load().oracle_partition_objects(_p(k), len(k), float(np.float32(position)), _p(perm))
+    return perm, split
 
 
 def rtsah(nodes):

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 
 #include "mh_device.h"
 
@@ -189,7 +190,9 @@ __device__ Partition partition_approx(const DevUpdate& u, int node, int b, int e
   }
   for (int i = b; i < e; ++i) {
     const float f = (obj_cen(u, ids[i], axis) - cb.lo[axis]) * scale;
-    int which = (f != f) ? 0 : (int)f;
+    // int(f) as the reference's x86 build converts it (cvttss2si): NaN and values outside
+    // the int range give INT_MIN, i.e. bin 0 after the clamp.
+    int which = (f >= -2147483648.f && f < 2147483648.f) ? (int)f : INT_MIN;
     which = which < 0 ? 0 : (which > kSamples - 1 ? kSamples - 1 : which);
     counts[which]++;
     extend(bins[which], obj_box(u, ids[i]));
