@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MH_ABI_VERSION 1
+#define MH_ABI_VERSION 2
 
 /* Status codes. */
 #define MH_OK 0
@@ -104,9 +104,15 @@ typedef struct mh_trace_opts {
 
 /* Build options. */
 typedef struct mh_build_opts {
-  int prim_kind; /* MH_PRIM_* [MH_PRIM_KENSLER_SHIRLEY] */
-  int rtsah;     /* 1 = compute isLeftCheaper (DynBVH.cc:1583-1678) [1] */
-  int threads;   /* host threads for the SAH build, 0 = all cores */
+  int prim_kind;   /* MH_PRIM_* [MH_PRIM_KENSLER_SHIRLEY] */
+  int rtsah;       /* 1 = compute isLeftCheaper (DynBVH.cc:1583-1678) [1] */
+  int threads;     /* host threads for the full SAH build, 0 = all cores */
+  /* 0 = the reference's default build: full-sweep SAH (partitionSAH), on the host.
+   * 1 = the reference's approximate build (USE_APPROXIMATE_BUILD, the
+   *     MANTA_USE_DYNBVH_APPROXIMATE option, include/CMakeLists.txt:134-147,
+   *     DynBVH.cc:1815-1822): 16-bin SAH at every node (partitionApproxSAH,
+   *     DynBVH.cc:2125-2275), built on the device; the same tree record for record. */
+  int approximate;
 } mh_build_opts;
 
 /* Library. */

@@ -83,7 +83,8 @@ class Camera(C.Structure):
 
 
 class BuildOpts(C.Structure):
-    _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int)]
+    _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int),
+                ("approximate", C.c_int)]
 
 
 class ServerOpts(C.Structure):

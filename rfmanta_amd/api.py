@@ -171,8 +171,10 @@ class Scene:
 
     # AccelerationStructure::rebuild
     def build(self, prim_kind: int = L.MH_PRIM_KENSLER_SHIRLEY, rtsah: bool = True,
-              threads: int = 0) -> "Scene":
-        o = L.BuildOpts(prim_kind, int(rtsah), threads)
+              threads: int = 0, approximate: bool = False) -> "Scene":
+        """DynBVH::rebuild: the full-sweep SAH on the host, or with approximate=True the
+        reference's approximate (binned) SAH build, run on the device."""
+        o = L.BuildOpts(prim_kind, int(rtsah), threads, int(approximate))
         L.check(L.load().mh_scene_build(self.h, C.byref(o)))
         return self
 
@@ -457,7 +459,7 @@ def build_dynbvh_host(verts: np.ndarray, idx: np.ndarray, rtsah: bool = True,
     ids = np.empty(T, np.int32)
     nodes = np.empty(2 * T, L.NODE_DTYPE)
     used = C.c_uint32()
-    o = L.BuildOpts(0, int(rtsah), threads)
+    o = L.BuildOpts(0, int(rtsah), threads, 0)
     L.check(L.load().mh_dynbvh_build_host(v.ctypes.data, len(v), i.ctypes.data, T, C.byref(o),
                                           ids.ctypes.data, nodes.ctypes.data, C.byref(used)))
     return nodes, ids, used.value

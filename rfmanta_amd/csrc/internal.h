@@ -84,11 +84,13 @@ struct mh_scene {
   // leaves to partition the object slots; trees loaded from elsewhere may not be.
   bool updatable = false;
   uint32_t n_inner = 0, n_leaves = 0, depth = 0;
+  uint32_t n_records = 0;  // records held (2T after a build; the file's count after a load)
 
   DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, leaves, levels;
   // rotating update (mh_scene_update): costs, subtree sizes, parents, object boxes and
   // centroids, depth work arrays, the depth histogram and counters
   DevBuf costs, ssize, parent, obox, ocen, anc0, dep0, anc1, dep1, hist, counters;
+  DevBuf logtab;  // RTSAH table logarithm (device build)
   std::vector<int64_t> level_off;  // inner records of depth d: levels[level_off[d], level_off[d+1])
 
   std::mutex spill_mu;  // guards the per-stream spill areas
@@ -102,7 +104,7 @@ struct mh_scene {
     return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
            d_base.bytes + d_idx.bytes + leaves.bytes + levels.bytes + costs.bytes + ssize.bytes +
            parent.bytes + obox.bytes + ocen.bytes + anc0.bytes + dep0.bytes + anc1.bytes +
-           dep1.bytes + hist.bytes + counters.bytes;
+           dep1.bytes + hist.bytes + counters.bytes + logtab.bytes;
   }
   // Traversal stack entries a single ray can hold at most (one push per level).
   uint32_t stack_bound(int /*layout*/) const {

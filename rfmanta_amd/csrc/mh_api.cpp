@@ -20,6 +20,7 @@
 
 #include "internal.h"
 #include "mh_device.h"
+#include "rtsah.h"
 #include "sah_builder.h"
 
 namespace mh_internal {
@@ -199,6 +200,7 @@ int upload(mh_scene* s) {
   s->n_leaves = (uint32_t)leaves.size();
   s->depth = max_depth;
   s->next_free = max_rec + 1;
+  s->n_records = nrec;
   // Leaf ranges lie inside [0, T); they partition it iff their sizes sum to T and no slot
   // is covered twice.
   bool partition = leaf_objects == T;
@@ -398,6 +400,190 @@ int mh_scene_destroy(mh_scene* scene) {
   return MH_OK;
 }
 
+// DynBVH::rebuild with USE_APPROXIMATE_BUILD, on the device (build.hip): object boxes,
+// the level loop over nodes of many objects (one counter read-back per level), one lane
+// per small node, then the reference numbering and the bottom-up passes.
+int build_device(mh_scene* s, int rtsah) {
+  using mh_internal::DevBuf;
+  const uint32_t T = s->nt;
+  const size_t nrec = 2 * (size_t)T;
+  const int64_t cap = (int64_t)T / (mh::kBuildSerial + 1) + 2;
+  const int64_t vcap = (int64_t)T / mh::kBuildVB + cap + 2;
+  int rc;
+  if ((rc = s->refs.reserve(nrec * sizeof(mh::RefNode))) ||
+      (rc = s->pairs.reserve(std::max<size_t>(1, nrec / 2) * sizeof(mh::PairNode))) ||
+      (rc = s->tris.reserve((size_t)T * 48)) || (rc = s->ids.reserve((size_t)T * 4)) ||
+      (rc = s->big.reserve((size_t)T * 4)) || (rc = s->leaves.reserve((size_t)T * 4)) ||
+      (rc = s->levels.reserve((size_t)T * 4)) || (rc = s->costs.reserve(nrec * 4)) ||
+      (rc = s->ssize.reserve(nrec * 4)) || (rc = s->parent.reserve(nrec * 4)) ||
+      (rc = s->obox.reserve((size_t)T * 24)) || (rc = s->ocen.reserve((size_t)T * 12)) ||
+      (rc = s->anc0.reserve(nrec * 4)) || (rc = s->dep0.reserve(nrec * 4)) ||
+      (rc = s->anc1.reserve(nrec * 4)) || (rc = s->dep1.reserve(nrec * 4)) ||
+      (rc = s->hist.reserve((mh::kMaxDepth + 2) * 4)) || (rc = s->counters.reserve(16)))
+    return rc;
+  if (!s->logtab.p) {
+    std::vector<float> tab(mh::rtsah::kLogTable);
+    mh::rtsah::fill_log_table(tab.data());
+    if ((rc = s->logtab.reserve(tab.size() * 4))) return rc;
+    HIP_OK(hipMemcpy(s->logtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+  }
+  // Work arrays of this build only.
+  DevBuf slots, trefs, seg0, seg1, small, big, plan, acck, accc, ctr, vcnt, gscan, scratch,
+      rootk, ic, rank, fin, rt;
+  if ((rc = slots.reserve((size_t)T * 48)) || (rc = trefs.reserve(nrec * 32)) ||
+      (rc = seg0.reserve(cap * 16)) || (rc = seg1.reserve(cap * 16)) ||
+      (rc = small.reserve((size_t)T * 16)) || (rc = big.reserve(cap * 16)) ||
+      (rc = plan.reserve(cap * 32)) || (rc = acck.reserve(cap * mh::kKeyFields * 8)) ||
+      (rc = accc.reserve(cap * mh::kCntFields * 4)) || (rc = ctr.reserve(16)) ||
+      (rc = vcnt.reserve(vcap * 8)) || (rc = gscan.reserve(2 * (vcap + 1) * 4)) ||
+      (rc = scratch.reserve(nrec * 4)) || (rc = rootk.reserve(48)) ||
+      (rc = ic.reserve(nrec * 4)) || (rc = rank.reserve(nrec * 4)) ||
+      (rc = fin.reserve(nrec * 4)) || (rc = rt.reserve(nrec * 8)))
+    return rc;
+  hipStream_t st = nullptr;
+  unsigned* cnt = s->counters.as<unsigned>();  // [0] records in use, [1] error
+  HIP_OK(hipMemsetAsync(cnt, 0, 16, st));
+  HIP_OK(hipMemsetAsync(ctr.p, 0, 16, st));
+  HIP_OK(hipMemsetAsync(rootk.p, 0xFF, 24, st));
+  HIP_OK(hipMemsetAsync(rootk.as<char>() + 24, 0, 24, st));
+
+  mh::DevBuild d{};
+  d.u.refs = trefs.as<mh::RefNode>();
+  d.u.costs = s->costs.as<float>();  // scratch until the bottom-up pass rewrites them
+  d.u.ssize = s->ssize.as<uint32_t>();
+  d.u.parent = s->parent.as<int32_t>();
+  d.u.ids = s->ids.as<int32_t>();
+  d.u.obox = s->obox.as<float>();
+  d.u.ocen = s->ocen.as<float>();
+  d.u.verts = s->d_verts.as<float>();
+  d.u.idx = s->d_idx.as<uint32_t>();
+  d.u.next_free = cnt;
+  d.u.err = reinterpret_cast<int*>(cnt + 1);
+  d.u.n_records = (int32_t)nrec;
+  d.u.n_objects = (int32_t)T;
+  d.u.large = (uint32_t)(nrec / 20);
+  d.slots = slots.as<float>();
+  d.verts = s->d_verts.as<float>();
+  d.n_verts = s->nv;
+  d.small = small.as<mh::BuildSeg>();
+  d.big = big.as<mh::BuildSeg>();
+  d.plan = plan.as<mh::BuildPlan>();
+  d.acck = acck.as<unsigned long long>();
+  d.accc = accc.as<unsigned>();
+  d.cap = cap;
+  d.ctr = ctr.as<unsigned>();
+  d.vcnt = vcnt.as<unsigned>();
+  d.gscan = gscan.as<unsigned>();
+  d.vcap = vcap;
+  d.lpos = scratch.as<int32_t>();
+  d.rpos = scratch.as<int32_t>() + T;
+  d.rootk = rootk.as<unsigned long long>();
+  HIP_OK((hipError_t)mh::launch_obj_bounds(d.u, st));
+  const bool serial_root = T <= (uint32_t)mh::kBuildSerial;
+  d.seg = seg0.as<mh::BuildSeg>();
+  HIP_OK((hipError_t)mh::build_launch_setup(d, serial_root ? 1 : 0, st));
+  mh::BuildSeg* cur = seg0.as<mh::BuildSeg>();
+  mh::BuildSeg* nxt = seg1.as<mh::BuildSeg>();
+  int64_t nseg = serial_root ? 0 : 1;
+  unsigned back[mh::kCtrCount];
+  int err = 0;
+  while (nseg > 0) {
+    d.seg = cur;
+    d.seg_next = nxt;
+    d.nseg = (int32_t)nseg;
+    HIP_OK(hipMemsetAsync(d.ctr + mh::kCtrNext, 0, 4, st));
+    const int64_t nvb_bound = (int64_t)T / mh::kBuildVB + nseg + 1;
+    HIP_OK((hipError_t)mh::build_launch_level(d, nvb_bound, st));
+    HIP_OK(hipMemcpyAsync(back, d.ctr, sizeof(back), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&err, cnt + 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (err) break;
+    nseg = back[mh::kCtrNext];
+    std::swap(cur, nxt);
+  }
+  HIP_OK(hipMemcpyAsync(back, d.ctr, sizeof(back), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(&err, cnt + 1, 4, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (err == 2) return fail(MH_E_INVALID, "non-finite vertex coordinate");
+  if (err == 3) return fail(MH_E_UNSUPPORTED, "a leaf of more than 32767 objects");
+  if (err) return fail(MH_E_STATE, "device build ran out of node records");
+  HIP_OK((hipError_t)mh::build_launch_serial(d, back[mh::kCtrSmall], back[mh::kCtrBig],
+                                             scratch.as<int32_t>(), st));
+  // Depths of the temporary tree, then its per-depth lists.
+  HIP_OK(hipMemsetAsync(s->hist.p, 0, (mh::kMaxDepth + 2) * 4, st));
+  HIP_OK((hipError_t)mh::launch_depths(d.u, s->anc0.as<int32_t>(), s->dep0.as<int32_t>(),
+                                       s->anc1.as<int32_t>(), s->dep1.as<int32_t>(),
+                                       s->hist.as<unsigned>(), st));
+  std::vector<unsigned> hist(mh::kMaxDepth + 2);
+  unsigned used[2];
+  HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(used, cnt, 8, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (used[1]) return fail(MH_E_STATE, "device build ran out of node records");
+  const unsigned deepest = hist[mh::kMaxDepth + 1];
+  if (deepest >= (unsigned)mh::kMaxDepth - 1)
+    return fail(MH_E_UNSUPPORTED, "tree deeper than %d levels", mh::kMaxDepth - 1);
+  std::vector<unsigned> cursor(mh::kMaxDepth + 1, 0u);
+  std::vector<int64_t> level_off(1, 0);
+  for (unsigned dd = 0; dd < deepest; ++dd) {
+    cursor[dd] = (unsigned)level_off.back();
+    level_off.push_back(level_off.back() + hist[dd]);
+  }
+  const int64_t n_leaves = hist[mh::kMaxDepth];
+  HIP_OK(hipMemcpyAsync(s->hist.p, cursor.data(), cursor.size() * 4, hipMemcpyHostToDevice, st));
+  HIP_OK((hipError_t)mh::launch_depth_scatter(d.u, s->dep0.as<int32_t>(), s->hist.as<unsigned>(),
+                                              s->levels.as<int32_t>(), s->leaves.as<int32_t>(),
+                                              st));
+  mh::DevFinish f{};
+  f.trefs = trefs.as<mh::RefNode>();
+  f.refs = s->refs.as<mh::RefNode>();
+  f.tcount = cnt;
+  f.ic = ic.as<int32_t>();
+  f.rank = rank.as<int32_t>();
+  f.fin = fin.as<int32_t>();
+  f.levels = s->levels.as<int32_t>();
+  f.leaves = s->leaves.as<int32_t>();
+  f.costs = s->costs.as<float>();
+  f.ssize = s->ssize.as<uint32_t>();
+  f.rt = rt.as<float>();
+  f.logtab = s->logtab.as<float>();
+  f.n_records = (int32_t)nrec;
+  f.large = (uint32_t)(nrec / 20);
+  f.rtsah = rtsah;
+  const int n_levels = (int)level_off.size() - 1;
+  HIP_OK((hipError_t)mh::build_launch_ranks(f, level_off.data(), n_levels, n_leaves, st));
+  mh::DevUpdate u = d.u;
+  u.refs = s->refs.as<mh::RefNode>();
+  u.pairs = s->pairs.as<mh::PairNode>();
+  u.big_count = s->big.as<int32_t>();
+  HIP_OK((hipError_t)mh::launch_rebuild_pairs(u, st));
+  const bool root_leaf = used[0] == 1;
+  if (root_leaf && T >= (uint32_t)mh::kLeafCountInline) {
+    const int32_t n = (int32_t)T;
+    HIP_OK(hipMemcpyAsync(s->big.p, &n, 4, hipMemcpyHostToDevice, st));
+  }
+  mh::DevRefit r{};
+  r.tris = s->tris.as<float>();
+  r.obj_ids = s->ids.as<int32_t>();
+  r.verts = s->d_verts.as<float>();
+  r.idx = s->d_idx.as<uint32_t>();
+  r.n_slots = (int32_t)T;
+  r.prim_kind = s->prim_kind;
+  HIP_OK((hipError_t)mh::launch_prim_setup(r, st));
+  HIP_OK(hipStreamSynchronize(st));
+  s->root_ref = root_leaf ? ~(int)std::min<uint32_t>(T, mh::kLeafCountInline) : 0;
+  s->next_free = used[0];
+  s->n_records = (uint32_t)nrec;
+  s->n_inner = (uint32_t)level_off.back();
+  s->n_leaves = (uint32_t)n_leaves;
+  s->depth = deepest;
+  s->level_off = std::move(level_off);
+  s->bvh = mh::DynBvh();
+  s->updatable = true;
+  s->built = true;
+  return MH_OK;
+}
+
 int mh_scene_build(mh_scene* s, const mh_build_opts* opts) {
   if (!s) return fail(MH_E_INVALID, "null scene");
   const int kind = opts ? opts->prim_kind : MH_PRIM_KENSLER_SHIRLEY;
@@ -407,6 +593,11 @@ int mh_scene_build(mh_scene* s, const mh_build_opts* opts) {
   const int threads = opts ? opts->threads : 0;
   int rc = ensure_init();
   if (rc) return rc;
+  if (opts && opts->approximate) {
+    s->built = s->updatable = false;
+    s->prim_kind = kind;
+    return build_device(s, rtsah);
+  }
   try {
     s->bvh = mh::build_dynbvh(s->verts.data(), s->nv, s->idx.data(), s->nt, threads);
     if (rtsah) mh::compute_rtsah(s->bvh.nodes, threads);
@@ -484,7 +675,7 @@ int mh_scene_export_dynbvh(const mh_scene* s, int32_t* ids, uint32_t* n_ids, voi
                            uint32_t* n_nodes) {
   if (!s || !n_ids || !n_nodes) return fail(MH_E_INVALID, "null argument");
   if (!s->built) return fail(MH_E_STATE, "scene has no tree");
-  const uint32_t ni = (uint32_t)s->bvh.ids.size(), nn = (uint32_t)s->bvh.nodes.size();
+  const uint32_t ni = s->nt, nn = s->n_records;
   if (!ids && !nodes) {
     *n_ids = ni;
     *n_nodes = nn;

@@ -159,6 +159,8 @@ int launch_obj_bounds(const DevUpdate& u, void* stream);
 int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
                        const int32_t* levels, const int64_t* level_off, int n_levels,
                        void* stream);
+// PairNodes and big leaf counts of every record pair in use, from the reference records.
+int launch_rebuild_pairs(const DevUpdate& u, void* stream);
 // Depth of every record in use; hist[d] inner records at depth d, hist[kMaxDepth] leaves,
 // hist[kMaxDepth + 1] the deepest depth (hist cleared by the caller). Depths in dep0.
 int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc1, int32_t* dep1,
@@ -200,5 +202,85 @@ int launch_secondary(const SecondaryArgs& a, int kind /*0 shadow, 1 bounce*/, vo
 
 // util.hip: streaming copy with 16-byte vector accesses (plus a byte tail).
 int launch_copy(void* dst, const void* src, size_t bytes, int cu_count, void* stream);
+
+
+// Device build (build.hip): DynBVH::rebuild with the approximate SAH at every node.
+constexpr int kBuildSerial = 128;  // nodes of at most this many objects: one lane each
+constexpr int kBuildVB = 2048;     // slots per virtual block of the level passes
+constexpr int kBuildBins = 16;     // BVH_num_samples (DynBVH.cc:34)
+
+// Accumulators of one level node, stored field-major ([field][node]) so the per-level
+// clear streams.
+enum : int {
+  kFcMin = 0,                    // 3 centroid min keys
+  kFcMax = 3,                    // 3 centroid max keys
+  kFbMin = 6,                    // 16 x 3 bin min keys
+  kFbMax = kFbMin + 3 * kBuildBins,   // 16 x 3 bin max keys
+  kFlMin = kFbMax + 3 * kBuildBins,   // 3 left-side min keys
+  kFlMax = kFlMin + 3,
+  kFrMin = kFlMax + 3,
+  kFrMax = kFrMin + 3,
+  kKeyFields = kFrMax + 3,
+};
+enum : int { kFcnt = 0, kFless = kBuildBins, kCntFields = kBuildBins + 1 };
+
+struct BuildSeg {  // a node of the current level: its temporary record and slot range
+  int32_t rec, b, e, vb0;
+};
+struct BuildPlan {
+  int32_t axis, leaf, split, nless;
+  float lo_c, scale, pos, width;
+};
+
+enum : int { kCtrNext = 0, kCtrSmall = 1, kCtrBig = 2, kCtrNvb = 3, kCtrCount = 4 };
+
+struct DevBuild {
+  DevUpdate u;                    // temporary records, ids, object boxes, counters
+  float* slots;                   // 12 floats per slot: centroid, id, box
+  const float* verts;
+  int64_t n_verts;
+  BuildSeg* seg;
+  BuildSeg* seg_next;
+  BuildSeg* small;
+  BuildSeg* big;
+  BuildPlan* plan;
+  unsigned long long* acck;       // [kKeyFields][cap]
+  unsigned* accc;                 // [kCntFields][cap]
+  int64_t cap;                    // capacity of the node lists
+  unsigned* ctr;                  // kCtr*
+  unsigned* vcnt;                 // per virtual block: left, right misplaced counts
+  unsigned* gscan;                // exclusive scans of vcnt (left at [0, nvb], right after)
+  int64_t vcap;                   // capacity of the virtual block arrays
+  int32_t* lpos;
+  int32_t* rpos;
+  unsigned long long* rootk;      // 6 keys: vertex box
+  int32_t nseg;
+};
+
+
+struct DevFinish {
+  const RefNode* trefs;    // temporary records
+  RefNode* refs;           // final records
+  const unsigned* tcount;  // records in use
+  int32_t* ic;             // inner records per temporary subtree
+  int32_t* rank;           // preorder rank of a temporary record among inner records
+  int32_t* fin;            // final number of a temporary record
+  int32_t* levels;
+  int32_t* leaves;
+  float* costs;
+  uint32_t* ssize;
+  float* rt;               // RTSAH pairs (c0, c1) per final record
+  const float* logtab;
+  int32_t n_records;
+  uint32_t large;
+  int rtsah;
+};
+
+int build_launch_setup(const DevBuild& d, int serial, void* stream);
+int build_launch_level(const DevBuild& d, int64_t nvb_bound, void* stream);
+int build_launch_serial(const DevBuild& d, int64_t n_small, int64_t n_big, int32_t* scratch,
+                        void* stream);
+int build_launch_ranks(const DevFinish& f, const int64_t* level_off, int n_levels,
+                       int64_t n_leaves, void* stream);
 
 }  // namespace mh

@@ -25,303 +25,13 @@
 #include <cfloat>
 #include <climits>
 
+#include "approx_sah.h"
 #include "mh_device.h"
 
 namespace mh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr float kCIsec = 1.f;  // BVH_C_isec (DynBVH.cc:29)
-constexpr float kCTrav = 1.f;  // BVH_C_trav (DynBVH.cc:30)
-constexpr int kSamples = 16;   // BVH_num_samples (DynBVH.cc:34)
-constexpr uint8_t kPending = 3;  // axis value of a record whose subtree is still to build
-
-struct Box {
-  float lo[3], hi[3];
-};
-
-__device__ __forceinline__ float mn(float a, float b) { return a < b ? a : b; }
-__device__ __forceinline__ float mx(float a, float b) { return a > b ? a : b; }
-__device__ __forceinline__ void reset(Box& b) {
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = FLT_MAX;
-    b.hi[k] = -FLT_MAX;
-  }
-}
-__device__ __forceinline__ void extend(Box& b, const Box& o) {
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = mn(b.lo[k], o.lo[k]);
-    b.hi[k] = mx(b.hi[k], o.hi[k]);
-  }
-}
-__device__ __forceinline__ void extend_pt(Box& b, const float* p) {
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = mn(b.lo[k], p[k]);
-    b.hi[k] = mx(b.hi[k], p[k]);
-  }
-}
-// BBox::computeArea (Core/Geometry/BBox.h:117-121): float arithmetic.
-__device__ __forceinline__ float area(const Box& b) {
-  const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
-  return 2 * (dx * dy + dy * dz + dx * dz);
-}
-__device__ __forceinline__ Box rbox(const RefNode& n) {
-  Box b;
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = n.box.lo[k];
-    b.hi[k] = n.box.hi[k];
-  }
-  return b;
-}
-__device__ __forceinline__ void set_box(RefNode& n, const Box& b) {
-  for (int k = 0; k < 3; ++k) {
-    n.box.lo[k] = b.lo[k];
-    n.box.hi[k] = b.hi[k];
-  }
-}
-__device__ __forceinline__ Box obj_box(const DevUpdate& u, int obj) {
-  Box b;
-  const float* p = u.obox + 6 * (size_t)obj;
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = p[k];
-    b.hi[k] = p[3 + k];
-  }
-  return b;
-}
-__device__ __forceinline__ float obj_cen(const DevUpdate& u, int obj, int axis) {
-  return u.ocen[3 * (size_t)obj + axis];
-}
-// Vector::indexOfMaxComponent (Core/Geometry/Vector.h:382-393)
-__device__ __forceinline__ int index_of_max(float a, float b, float c) {
-  if (a > b) return a > c ? 0 : 2;
-  return b > c ? 1 : 2;
-}
-// BBox::longestAxis (Core/Geometry/BBox.h:123-129)
-__device__ __forceinline__ int longest_axis(const Box& b) {
-  return index_of_max(b.hi[0] - b.lo[0], b.hi[1] - b.lo[1], b.hi[2] - b.lo[2]);
-}
-
-struct Partition {
-  Box l, r;
-  int axis, split;
-};
-
-// partition2Objs (DynBVH.cc:2277-2308)
-__device__ Partition partition2(const DevUpdate& u, int node, int b) {
-  Partition p;
-  reset(p.l);
-  reset(p.r);
-  int32_t* ids = u.ids;
-  const float nodeArea = area(rbox(u.refs[node]));
-  const float leftArea = area(obj_box(u, ids[b]));
-  const float rightArea = area(obj_box(u, ids[b + 1]));
-  const float leafCost = kCIsec * 2;
-  const float newCost = kCTrav + (leftArea * kCIsec + rightArea * kCIsec) / nodeArea;
-  if (newCost < leafCost) {
-    const Box nb = rbox(u.refs[node]);
-    const int axis = index_of_max(nb.hi[0] - nb.lo[0], nb.hi[1] - nb.lo[1], nb.hi[2] - nb.lo[2]);
-    p.axis = axis;
-    p.split = b + 1;
-    if (obj_box(u, ids[b]).lo[axis] > obj_box(u, ids[b + 1]).lo[axis]) {
-      const int t = ids[b];
-      ids[b] = ids[b + 1];
-      ids[b + 1] = t;
-    }
-    p.l = obj_box(u, ids[b]);
-    p.r = obj_box(u, ids[b + 1]);
-    return p;
-  }
-  p.axis = -1;
-  p.split = 0;
-  return p;
-}
-
-// partitionObjects (DynBVH.cc:1369-1396)
-__device__ int partition_objects(const DevUpdate& u, int b, int e, int axis, float position) {
-  int32_t* ids = u.ids;
-  int first = b - 1, last = e;
-  for (;;) {
-    for (++first; first < last; first++)
-      if (obj_cen(u, ids[first], axis) >= position) break;
-    for (--last; first < last; --last)
-      if (obj_cen(u, ids[last], axis) < position) break;
-    if (first < last) {
-      const int t = ids[first];
-      ids[first] = ids[last];
-      ids[last] = t;
-    } else {
-      return first;
-    }
-  }
-}
-
-// partitionApproxSAH (DynBVH.cc:2125-2275); LONGEST_AXIS is defined (DynBVH.cc:39), so
-// only the centroid box's longest axis is binned.
-__device__ Partition partition_approx(const DevUpdate& u, int node, int b, int e) {
-  Partition p;
-  reset(p.l);
-  reset(p.r);
-  const int n = e - b;
-  if (n == 1) {
-    p.axis = -1;
-    p.split = -1;
-    return p;
-  }
-  if (n == 2) return partition2(u, node, b);
-  const int32_t* ids = u.ids;
-  Box cb;
-  reset(cb);
-  for (int i = b; i < e; ++i) {
-    const float c[3] = {obj_cen(u, ids[i], 0), obj_cen(u, ids[i], 1), obj_cen(u, ids[i], 2)};
-    extend_pt(cb, c);
-  }
-  const float inv_parent_area = (float)(1.0 / (double)area(rbox(u.refs[node])));
-  float best_cost = n * kCIsec;
-  int best_axis = -1;
-  float best_pos = FLT_MAX;
-  const int axis = longest_axis(cb);
-  const float width = cb.hi[axis] - cb.lo[axis];
-  const float scale = ((float)kSamples / width) * .999f;
-  Box bins[kSamples];
-  int counts[kSamples];
-  for (int i = 0; i < kSamples; ++i) {
-    reset(bins[i]);
-    counts[i] = 0;
-  }
-  for (int i = b; i < e; ++i) {
-    const float f = (obj_cen(u, ids[i], axis) - cb.lo[axis]) * scale;
-    // int(f) as the reference's x86 build converts it (cvttss2si): NaN and values outside
-    // the int range give INT_MIN, i.e. bin 0 after the clamp.
-    int which = (f >= -2147483648.f && f < 2147483648.f) ? (int)f : INT_MIN;
-    which = which < 0 ? 0 : (which > kSamples - 1 ? kSamples - 1 : which);
-    counts[which]++;
-    extend(bins[which], obj_box(u, ids[i]));
-  }
-  float left_areas[kSamples], right_areas[kSamples];
-  int left_counts[kSamples], right_counts[kSamples];
-  Box lb, rb;
-  reset(lb);
-  reset(rb);
-  int nl = 0, nr = n;
-  for (int i = 0; i < kSamples; ++i) {
-    nl += counts[i];
-    nr -= counts[i];
-    extend(lb, bins[i]);
-    left_counts[i] = nl;
-    right_counts[i] = nr;
-    left_areas[i] = area(lb);
-  }
-  for (int i = kSamples - 1; i >= 0; --i) {
-    right_areas[i] = (i == kSamples - 1) ? FLT_MAX : area(rb);
-    extend(rb, bins[i]);
-  }
-  for (int i = 0; i < kSamples - 1; ++i) {
-    const float cost = (left_areas[i] * left_counts[i] + right_areas[i] * right_counts[i]) *
-                           inv_parent_area * kCIsec + kCTrav;
-    if (cost < best_cost) {
-      best_cost = cost;
-      best_axis = axis;
-      best_pos = cb.lo[axis] + width * (static_cast<float>(i + 1) / kSamples);
-    }
-  }
-  p.axis = best_axis;
-  if (p.axis != -1) {
-    int middle = partition_objects(u, b, e, p.axis, best_pos);
-    if (middle == b || middle == e) {
-      if (n < 3) {
-        p.axis = -1;
-        p.split = 0;
-        return p;
-      }
-      middle = b + n / 2;
-    }
-    for (int i = b; i < middle; ++i) extend(p.l, obj_box(u, ids[i]));
-    for (int i = middle; i < e; ++i) extend(p.r, obj_box(u, ids[i]));
-    p.split = middle;
-    return p;
-  }
-  p.split = 0;
-  return p;
-}
-
-// Turns `node` into an inner node over a new record pair whose records wait (axis =
-// kPending) to be built over [b, split) and [split, e). False if the record array is full.
-__device__ bool make_inner(const DevUpdate& u, int node, const Partition& p, int b, int e) {
-  const unsigned spot = atomicAdd(u.next_free, 2u);
-  if (spot + 1 >= (unsigned)u.n_records) {
-    atomicExch(u.err, 1);
-    return false;
-  }
-  RefNode& nd = u.refs[node];
-  nd.child = (int32_t)spot;
-  nd.children = 0;
-  nd.axis = (uint8_t)p.axis;
-  RefNode& c0 = u.refs[spot];
-  RefNode& c1 = u.refs[spot + 1];
-  set_box(c0, p.l);
-  set_box(c1, p.r);
-  c0.child = b;
-  c0.children = (int16_t)(p.split - b);
-  c0.axis = kPending;
-  c1.child = p.split;
-  c1.children = (int16_t)(e - p.split);
-  c1.axis = kPending;
-  u.parent[spot] = node;
-  u.parent[spot + 1] = node;
-  return true;
-}
-
-// DynBVH::build(leaf, begin, end, useApproximateBuild = true, nodeAlreadyExists = true):
-// depth-first over the new subtree without a stack, using parent links.
-__device__ void split_leaf(const DevUpdate& u, int root) {
-  RefNode* refs = u.refs;
-  const int b0 = refs[root].child, e0 = b0 + refs[root].children;
-  const Partition p0 = partition_approx(u, root, b0, e0);
-  if (p0.axis == -1) return;  // the leaf stays as it is
-  if (!make_inner(u, root, p0, b0, e0)) return;
-  int cur = refs[root].child;
-  for (;;) {
-    // cur waits to be built over [child, child + children)
-    const int b = refs[cur].child, e = b + refs[cur].children;
-    const Partition p = partition_approx(u, cur, b, e);
-    bool descended = false;
-    if (p.axis == -1) {
-      RefNode& nd = refs[cur];
-      nd.child = b;
-      nd.children = (int16_t)(e - b);
-      nd.axis = 0;
-      u.costs[cur] = nd.children * kCIsec;
-      u.ssize[cur] = 1;
-      for (int i = b + 1; i < e; ++i) {  // std::sort of the leaf's object ids
-        const int32_t v = u.ids[i];
-        int j = i - 1;
-        while (j >= b && u.ids[j] > v) {
-          u.ids[j + 1] = u.ids[j];
-          --j;
-        }
-        u.ids[j + 1] = v;
-      }
-    } else if (make_inner(u, cur, p, b, e)) {
-      cur = refs[cur].child;
-      descended = true;
-    } else {
-      return;  // record array full: reported through err
-    }
-    if (descended) continue;
-    // climb: finish parents whose right subtree is done, move to the next right sibling
-    for (;;) {
-      const int par = u.parent[cur];
-      if (cur == refs[par].child) {
-        cur = cur + 1;
-        break;
-      }
-      u.ssize[par] = 1 + u.ssize[refs[par].child] + u.ssize[refs[par].child + 1];
-      if (par == root) return;
-      cur = par;
-    }
-  }
-}
-
 __device__ __forceinline__ void swap_rec(const DevUpdate& u, int a, int b) {
   int4* pa = reinterpret_cast<int4*>(u.refs + a);
   int4* pb = reinterpret_cast<int4*>(u.refs + b);
@@ -679,6 +389,12 @@ int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leav
       hipLaunchKernelGGL(k_upd_level, dim3(grid(n)), dim3(kBlock), 0, s, u, levels + first, n);
   }
   hipLaunchKernelGGL(k_rebuild_pairs, dim3(grid(u.n_records)), dim3(kBlock), 0, s, u);
+  return (int)hipGetLastError();
+}
+
+int launch_rebuild_pairs(const DevUpdate& u, void* stream) {
+  hipLaunchKernelGGL(k_rebuild_pairs, dim3(grid(u.n_records)), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), u);
   return (int)hipGetLastError();
 }
 

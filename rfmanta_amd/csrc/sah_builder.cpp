@@ -15,6 +15,8 @@
 
 #include "sah_builder.h"
 
+#include "rtsah.h"
+
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
@@ -257,171 +259,22 @@ class Splitter {
 // ---------------------------------------------------------------------------
 // RTSAH child order (DynBVH.cc:1594-1678, Model/Groups/TreeTraversalProbability.*)
 // ---------------------------------------------------------------------------
-class FastLog {  // table logarithm, Core/Math/ExponSSE.h:20-73 at precision 15
- public:
-  FastLog() : tab_(1u << kBits) {
-    float x = 1.0f + (1.0f / (float)(1 << (kBits + 1)));
-    for (uint32_t i = 0; i < (1u << kBits); ++i) {
-      tab_[i] = logf(x) / 0.69314718055995f;
-      x += 1.0f / (float)(1 << kBits);
-    }
-  }
-  float operator()(float val) const {
-    int32_t w;
-    std::memcpy(&w, &val, 4);
-    const int ex = ((w >> 23) & 255) - 127;
-    const int mi = (w & 0x7FFFFF) >> (23 - kBits);
-    return ((float)ex + tab_[mi]) * 0.69314718055995f;
-  }
-
- private:
-  static const int kBits = 15;
-  std::vector<float> tab_;
-};
-
-inline float approx_atan2(float y, float x) {  // Core/Math/Trig.h:153-178
-  const float half_pi = (float)M_PI_2, pi = (float)M_PI;
-  if (x == 0.0f) return y > 0.0f ? half_pi : (y == 0.0f ? 0.0f : -half_pi);
-  const float z = y / x;
-  float r;
-  if (fabsf(z) < 1.0f) {
-    r = z / (1.0f + 0.28f * z * z);
-    if (x < 0.0f) return y < 0.0f ? r - pi : r + pi;
-  } else {
-    r = half_pi - z / (z * z + 0.28f);
-    if (y < 0.0f) return r - pi;
-  }
-  return r;
-}
-
-class FormFactors {
- public:
-  // Probabilities that a ray through `parent` crosses both children (lr), only the
-  // left (jl) or only the right (jr); TreeTraversalProbability.cc:3-95.
-  void probabilities(const Aabb& parent, const Aabb& L, const Aabb& R, float& lr, float& jl,
-                     float& jr) const {
-    float dl[3], dr[3];
-    for (int k = 0; k < 3; ++k) {
-      dl[k] = L.hi[k] - L.lo[k];
-      dr[k] = R.hi[k] - R.lo[k];
-    }
-    const bool thinL[3] = {dl[1] == 0 || dl[2] == 0, dl[0] == 0 || dl[2] == 0,
-                           dl[0] == 0 || dl[1] == 0};
-    const bool thinR[3] = {dr[1] == 0 || dr[2] == 0, dr[0] == 0 || dr[2] == 0,
-                           dr[0] == 0 || dr[1] == 0};
-    lr = 0;
-    for (int i = 0; i < 6; ++i) {
-      const int fi = i % 3;
-      if (thinL[fi]) continue;
-      if (i < 3 ? (L.hi[fi] >= R.hi[fi]) : (L.lo[fi] <= R.lo[fi])) continue;
-      for (int j = 0; j < 6; ++j) {
-        if (i == j) continue;
-        const int fj = j % 3;
-        if (thinR[fj]) continue;
-        Aabb pl = L, pr = R;
-        if (i < 3) pl.lo[fi] = pl.hi[fi]; else pl.hi[fi] = pl.lo[fi];
-        if (j < 3) pr.lo[fj] = pr.hi[fj]; else pr.hi[fj] = pr.lo[fj];
-        if (fi == fj) {
-          if (i < 3 ? (pl.hi[fi] >= pr.lo[fi]) : (pl.lo[fi] <= pr.hi[fi])) continue;
-          lr += parallel_ff(pl, pr, fi);
-        } else {
-          if (j < 3) {
-            if (pl.hi[fj] <= pr.hi[fj]) continue;
-            pl.lo[fj] = pick_hi(pl.lo[fj], pr.hi[fj]);
-          } else {
-            if (pl.lo[fj] >= pr.lo[fj]) continue;
-            pl.hi[fj] = pick_lo(pl.hi[fj], pr.lo[fj]);
-          }
-          if (i < 3)
-            pr.lo[fi] = pick_hi(pr.lo[fi], pl.hi[fi]);
-          else
-            pr.hi[fi] = pick_lo(pr.hi[fi], pl.lo[fi]);
-          lr += perpendicular_ff(pl, pr, fi, fj);
-        }
-      }
-    }
-    const float pinv = inv_area(parent);
-    lr += overlap_area(L, R);
-    lr *= pinv;
-    jl = (float)((double)box_area(L) * (double)pinv - (double)lr);
-    jr = (float)((double)box_area(R) * (double)pinv - (double)lr);
-  }
-
- private:
-  static float sgn(int i) { return i % 2 == 1 ? -1.0f : 1.0f; }
-
-  float perp_term(float v, float z, float x, float y, float z0, float x1) const {
-    const float a = y - v, b = z0 - z, c = x - x1;
-    const float aa = a * a, cc = c * c, bb = b * b;
-    const float root = sqrtf(cc + bb);
-    return a * root * approx_atan2(a, root) + .25f * (aa - bb - cc) * log_(aa + bb + cc);
-  }
-  float perpendicular_ff(const Aabb& b0, const Aabb& b1, int d1, int d2) const {
-    const int ax = (~(unsigned)(d1 + d2)) & 3;
-    const float v[2] = {b1.lo[ax], b1.hi[ax]}, y[2] = {b0.lo[ax], b0.hi[ax]};
-    const float x[2] = {b0.lo[d2], b0.hi[d2]}, z[2] = {b1.lo[d1], b1.hi[d1]};
-    const float x1 = b1.lo[d2], z0 = b0.lo[d1];
-    float F = 0;
-    for (int i = 1; i <= 2; ++i)
-      for (int j = 1; j <= 2; ++j)
-        for (int k = 1; k <= 2; ++k)
-          for (int m = 1; m <= 2; ++m)
-            F += perp_term(v[i - 1], z[j - 1], x[k - 1], y[m - 1], z0, x1) * sgn(i + j + k + m);
-    F *= (float)M_1_PI;
-    return fabsf(F);
-  }
-  float par_term(float u, float v, float x, float y, float zi) const {
-    const float a = (x - u) * zi, b = (y - v) * zi;
-    const float aa = a * a, bb = b * b;
-    const float ra = sqrtf(1 + aa), rb = sqrtf(1 + bb);
-    return b * ra * approx_atan2(b, ra) + a * rb * approx_atan2(a, rb) - 0.5f * log_(1 + aa + bb);
-  }
-  float parallel_ff(const Aabb& b0, const Aabb& b1, int flat) const {
-    const int d1 = (flat + 1) % 3, d2 = (flat + 2) % 3;
-    const float y[2] = {b0.lo[d1], b0.hi[d1]}, v[2] = {b1.lo[d1], b1.hi[d1]};
-    const float x[2] = {b0.lo[d2], b0.hi[d2]}, u[2] = {b1.lo[d2], b1.hi[d2]};
-    const float z = b0.lo[flat] - b1.lo[flat];
-    const float zi = 1.0f / z;
-    float F = 0;
-    for (int i = 1; i <= 2; ++i)
-      for (int j = 1; j <= 2; ++j)
-        for (int k = 1; k <= 2; ++k)
-          for (int m = 1; m <= 2; ++m)
-            F += par_term(u[i - 1], v[j - 1], x[k - 1], y[m - 1], zi) * sgn(i + j + k + m);
-    F *= z * z * (float)M_1_PI;
-    return fabsf(F);
-  }
-  static float overlap_area(const Aabb& a, const Aabb& b) {
-    Aabb o;
-    for (int k = 0; k < 3; ++k) {
-      o.lo[k] = pick_hi(a.lo[k], b.lo[k]);
-      o.hi[k] = pick_lo(a.hi[k], b.hi[k]);
-    }
-    if (o.hi[0] - o.lo[0] < 0 || o.hi[1] - o.lo[1] < 0 || o.hi[2] - o.lo[2] < 0) return 0;
-    return box_area(o);
-  }
-  FastLog log_;
-};
-
-struct CostPair {
-  float c0, c1;
-};
-
 class RtsahPass {
  public:
-  RtsahPass(std::vector<RefNode>& n, int threads) : nodes_(n), max_threads_(threads) {}
+  RtsahPass(std::vector<RefNode>& n, int threads)
+      : nodes_(n), max_threads_(threads), log_(rtsah::kLogTable) {
+    rtsah::fill_log_table(log_.data());
+  }
   void run() {
     live_.store(1);
     visit(0, 0);
   }
 
  private:
-  CostPair visit(int id, int depth) {
+  rtsah::CostPair visit(int id, int depth) {
     RefNode& nd = nodes_[id];
-    if (nd.leaf()) return CostPair{nd.children * 3.0f + 1.0f, 0.0f};
-    float Plr, Pjl, Pjr;
-    ff_.probabilities(nd.box, nodes_[nd.child].box, nodes_[nd.child + 1].box, Plr, Pjl, Pjr);
-    CostPair L, R;
+    if (nd.leaf()) return rtsah::leaf_cost(nd.children);
+    rtsah::CostPair L, R;
     if (depth < 12 && live_.fetch_add(1) < max_threads_) {
       std::thread th([&]() {
         R = visit(nd.child + 1, depth + 1);
@@ -434,22 +287,19 @@ class RtsahPass {
       L = visit(nd.child, depth + 1);
       R = visit(nd.child + 1, depth + 1);
     }
-    const float Pl = Pjl + Plr, Pr = Pjr + Plr;
-    const float Pe = 1 - Pjr - Pjl - Plr;
-    const float lc = Pl * L.c0 + (Pjr + Plr * L.c1) * R.c0 + (1 + Pl * L.c1 + (1 - Pl)) * 1.0f;
-    const float rc = Pr * R.c0 + (Pjl + Plr * R.c1) * L.c0 + (1 + Pr * R.c1 + (1 - Pr)) * 1.0f;
-    float c1 = Pjl * L.c1 + Pjr * R.c1 + Plr * L.c1 * R.c1 + Pe;
-    c1 = c1 <= 0.0f ? 0.0f : (c1 >= 1.0f ? 1.0f : c1);
-    if (lc < rc)
+    bool left_cheaper;
+    const rtsah::CostPair c = rtsah::combine(log_.data(), nd.box, nodes_[nd.child].box,
+                                             nodes_[nd.child + 1].box, L, R, left_cheaper);
+    if (left_cheaper)
       nd.bits |= 1;
     else
       nd.bits &= (uint8_t)~1;
-    return CostPair{pick_lo(lc, rc), c1};
+    return c;
   }
   std::vector<RefNode>& nodes_;
   int max_threads_;
   std::atomic<int> live_{1};
-  FormFactors ff_;
+  std::vector<float> log_;
 };
 
 }  // namespace

@@ -202,7 +202,7 @@ void HipDynBVH::ensureScene() {
 void HipDynBVH::rebuildSerial() {
   if (group_changed_ || !scene_) {
     ensureScene();
-    mh_build_opts o{prim_kind_, 1, 0};
+    mh_build_opts o{prim_kind_, 1, 0, 0};
     if (mh_scene_build(scene_, &o) != MH_OK) fail("rebuild");
     group_changed_ = false;
     batcher_.reset(new PacketBatcher(scene_, max_packets_, wait_us_));

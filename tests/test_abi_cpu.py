@@ -28,7 +28,7 @@ def test_every_declared_symbol_is_exported(mh):
 
 
 def test_abi_version(mh):
-    assert mh.mh_abi_version() == 1
+    assert mh.mh_abi_version() == 2
 
 
 def test_node_record_layout():
