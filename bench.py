@@ -251,6 +251,12 @@ def main() -> int:
     }
     frame.close()
 
+    if "build" not in skip:
+        out["build"] = bench_build(api, L, verts, idx, build_s,
+                                   {"coherent": (d_prim, h_prim, False, n_prim),
+                                    "incoherent": (d_b, h_b, False, n_b),
+                                    "shadow_anyhit": (d_s, h_s, True, n_s)}, args)
+
     if "c2" not in skip:
         out["c2_batching"] = bench_c2(api, L, scenes, scene, verts, idx, args)
     if "c3" not in skip:
@@ -268,6 +274,39 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     scene.close()
     return 0
+
+
+def bench_build(api, L, verts, idx, host_build_s, sets, args) -> dict:
+    """Time to a traceable tree: the host full-sweep SAH build (the reference's default)
+    against the device build of the reference's approximate (binned) SAH variant, and the
+    plain kernel's rate on the approximate tree for the C1 ray sets."""
+    sc = api.Scene(verts, idx)
+    times = []
+    for _ in range(3):
+        api.sync()
+        t = time.perf_counter()
+        sc.build(approximate=True)
+        api.sync()
+        times.append(1e3 * (time.perf_counter() - t))
+    st = sc.stats()
+    opts = api.trace_opts(1)
+    rates = {}
+    for name, (dr, dh, any_hit, n) in sets.items():
+        ms = C.c_float()
+        r, h = dr.struct(), dh.struct()
+        sc.trace(dr, dh, any_hit=any_hit, opts=opts)
+        L.check(L.load().mh_time_trace(sc.h, C.byref(r), C.byref(h), dr.n,
+                                       L.MH_ANY_HIT if any_hit else 0, C.byref(opts), None,
+                                       args.steps, C.byref(ms)))
+        rates[f"{name}_mrays"] = round(n / (ms.value / args.steps * 1e-3) / 1e6, 1)
+    sc.close()
+    res = {"tris": len(idx), "host_full_sah_s": round(host_build_s, 3),
+           "device_approx_ms": round(float(np.median(times)), 2),
+           "device_approx_ms_all": [round(t, 2) for t in times],
+           "approx_nodes": st["nodes"], "approx_depth": st["depth"],
+           "approx_tree_plain": rates}
+    log(f"build: {res}")
+    return res
 
 
 def bench_c2(api, L, scenes, scene, verts, idx, args) -> dict:
@@ -427,9 +466,19 @@ def bench_c4(api, scenes, args) -> dict:
     sc = api.Scene(v, i)
     nv, nt = len(v), len(i)
     del v, i
+    approx_ms = []
+    for _ in range(2):  # the device build of the approximate-SAH variant, for comparison
+        api.sync()
+        t = time.perf_counter()
+        sc.build(approximate=True)
+        api.sync()
+        approx_ms.append(1e3 * (time.perf_counter() - t))
+    t = time.perf_counter()
     sc.build()
+    host_s = time.perf_counter() - t
     st = sc.stats()
-    log(f"C4: {nt} tris generated+built in {time.time() - t0:.1f}s, {st}")
+    log(f"C4: {nt} tris generated+built in {time.time() - t0:.1f}s (host full SAH "
+        f"{host_s:.2f}s, device approximate {approx_ms} ms), {st}")
     srv = api.Server([sc], hbm_budget=2 << 30, host_budget=1 << 30, chunk_rays=1 << 22,
                      shadows=True, bounces=True, light=(2.0, 2.0, -2.0))
     cam = api.camera((0.0, 0.7, -0.9), (0.0, 0.0, 0.0), (0, 1, 0), 30.0, 1920, 1080, seed=4)
@@ -465,6 +514,8 @@ def bench_c4(api, scenes, args) -> dict:
     # nodes reading two child boxes) plus the displacement pass (read base, write verts).
     refit_bytes = nt * 48 + st["nodes"] * 32 + st["inner"] * 48 + nv * 24
     res = {"frames": args.c4_frames, "tris": nt, "rays_per_frame": rays // args.c4_frames,
+           "host_full_sah_build_s": round(host_s, 2),
+           "device_approx_build_ms": round(min(approx_ms), 2),
            "primary_hits_frame0": hits,
            "frame_ms_mean": round(float(np.mean(frame_ms)), 3),
            "frame_ms_p99": round(float(np.percentile(frame_ms, 99)), 3),

@@ -768,10 +768,12 @@ __global__ __launch_bounds__(kBlock) void k_emit(DevFinish f) {
     f.refs[x] = b;
     return;
   }
+  const int32_t to = f.fin[x];
+  if (to < 0 || to >= f.n_records) return;  // every record in use is numbered; a guard only
   RefNode t = f.trefs[x];
   t.bits = 0;
   if (t.children == 0) t.child = 2 * f.rank[x] + 1;
-  f.refs[f.fin[x]] = t;
+  f.refs[to] = t;
 }
 __global__ __launch_bounds__(kBlock) void k_map_lists(DevFinish f, int64_t n_inner,
                                                       int64_t n_leaves) {

@@ -534,6 +534,7 @@ int build_device(mh_scene* s, int rtsah) {
   HIP_OK((hipError_t)mh::launch_depth_scatter(d.u, s->dep0.as<int32_t>(), s->hist.as<unsigned>(),
                                               s->levels.as<int32_t>(), s->leaves.as<int32_t>(),
                                               st));
+  HIP_OK(hipMemsetAsync(fin.p, 0xFF, nrec * 4, st));
   mh::DevFinish f{};
   f.trefs = trefs.as<mh::RefNode>();
   f.refs = s->refs.as<mh::RefNode>();
