@@ -178,6 +178,34 @@ int mh_scene_vertices_device(mh_scene* scene, float** verts);
  * from the base positions given at create time, then no refit. */
 int mh_scene_displace_sine(mh_scene* scene, float amp, float freq, float phase, void* stream);
 
+/* Per-vertex shading attributes of the mesh, Mesh::vertexNormals + normal_indices and
+ * Mesh::texCoords + texture_indices (Model/Groups/Mesh.h:28-31): n_normals normals
+ * (3 floats each) indexed by normal_idx (3 per triangle), n_texcoords texture coordinates
+ * (3 floats each) indexed by tex_idx (3 per triangle; a triangle whose first entry is
+ * MH_NO_TEXTURE_INDEX, Mesh::kNoTextureIndex, has none). A null array pair clears that
+ * attribute. Host arrays, copied to the device. */
+#define MH_NO_TEXTURE_INDEX 0xFFFFFFFFu
+int mh_scene_set_attributes(mh_scene* scene, const float* normals, uint32_t n_normals,
+                            const uint32_t* normal_idx, const float* texcoords,
+                            uint32_t n_texcoords, const uint32_t* tex_idx);
+
+/* Outputs of mh_shade, device arrays of n floats; any may be NULL. */
+typedef struct mh_shading {
+  float *px, *py, *pz; /* hit position o + d * t */
+  float *nx, *ny, *nz; /* shading normal */
+  float *gx, *gy, *gz; /* geometric normal */
+  float *tu, *tv, *tw; /* texture coordinates */
+} mh_shading;
+
+/* What Manta's materials read from a hit, for every hit ray of a traced set, on the
+ * device: RayPacket::computeHitPositions (Interface/RayPacket.cc:114-146), computeNormal
+ * and computeGeometricNormal (KenslerShirleyTriangle.cc:381-420, WaldTriangle.cc:558-609,
+ * by the scene's triangle kind) and computeTexCoords2 (KenslerShirleyTriangle.cc:85-123,
+ * WaldTriangle.cc:131-169). hits must hold prim, t, u and v of a closest-hit trace of the
+ * same rays; rays that missed get zeros. Asynchronous on `stream`. */
+int mh_shade(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
+             const mh_shading* out, void* stream);
+
 /* DynBVH::intersect (DynBVH.cc:101-404) on n rays resident in device memory.
  * Asynchronous on `stream` (NULL = the null stream). */
 int mh_trace(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,

@@ -1442,6 +1442,69 @@ int oracle_build_approx(const float* verts, int nverts, const uint32_t* idx, int
   return next_free;  // every allocated record is in the tree
 }
 
+// What shading reads from a hit, per ray: the hit position
+// (RayPacket::actualComputeHitPositions, Interface/RayPacket.cc:114-146), the shading
+// normal (KenslerShirleyTriangle::computeNormal, KenslerShirleyTriangle.cc:381-405;
+// WaldTriangle::computeNormal, WaldTriangle.cc:558-590), the geometric normal
+// (KenslerShirleyTriangle.cc:407-420; WaldTriangle.cc:592-609) and the texture coordinates
+// (computeTexCoords2, KenslerShirleyTriangle.cc:85-123; WaldTriangle.cc:131-169). u and v
+// are the scratchpad weights of vertices 1 and 2. normals/nidx (Mesh::vertexNormals and
+// normal_indices) and tex/tidx (Mesh::texCoords and texture_indices) may be null. out
+// holds 12 floats per ray: position, normal, geometric normal, texture coordinates; rays
+// that missed (prim < 0) get zeros.
+void oracle_shade(const float* verts, int nverts, const uint32_t* idx, int ntris, int prim_kind,
+                  const float* normals, const uint32_t* nidx, const float* tex,
+                  const uint32_t* tidx, const float* const* ray6, const int32_t* prim,
+                  const float* t, const float* u, const float* v, int64_t n, float* out) {
+  Mesh m{verts, nverts, idx, ntris};
+  for (int64_t r = 0; r < n; ++r) {
+    float* o = out + 12 * r;
+    for (int k = 0; k < 12; ++k) o[k] = 0.f;
+    const int id = prim[r];
+    if (id < 0) continue;
+    for (int j = 0; j < 3; ++j) o[j] = ray6[j][r] + ray6[3 + j][r] * t[r];
+    const float a = u[r], b = v[r];
+    const float c = (float)(1.0 - a - b);  // Real c = (1.0 - a - b): double, then narrowed
+    const float* p0 = m.vtx(id, 0);
+    const float* p1 = m.vtx(id, 1);
+    const float* p2 = m.vtx(id, 2);
+    float face[3];
+    if (prim_kind == 1) {
+      const Wald w = waldSetup(p0, p1, p2);
+      const int ku = (w.k == 2) ? 0 : w.k + 1, kv = (w.k == 0) ? 2 : w.k - 1;
+      face[w.k] = w.n_k;
+      face[ku] = w.n_k * w.n_u;
+      face[kv] = w.n_k * w.n_v;
+    } else {  // Cross(p1 - p0, p2 - p0)
+      const float e0[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+      const float e1[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+      face[0] = e0[1] * e1[2] - e0[2] * e1[1];
+      face[1] = e0[2] * e1[0] - e0[0] * e1[2];
+      face[2] = e0[0] * e1[1] - e0[1] * e1[0];
+    }
+    for (int j = 0; j < 3; ++j) o[6 + j] = face[j];
+    if (normals && nidx) {  // (n1 * a) + (n2 * b) + (n0 * c)
+      const float* n0 = normals + 3 * (size_t)nidx[3 * (size_t)id + 0];
+      const float* n1 = normals + 3 * (size_t)nidx[3 * (size_t)id + 1];
+      const float* n2 = normals + 3 * (size_t)nidx[3 * (size_t)id + 2];
+      for (int j = 0; j < 3; ++j) o[3 + j] = (n1[j] * a + n2[j] * b) + n0[j] * c;
+    } else {
+      for (int j = 0; j < 3; ++j) o[3 + j] = face[j];
+    }
+    const uint32_t t0 = (tex && tidx) ? tidx[3 * (size_t)id] : 0xFFFFFFFFu;
+    if (t0 == 0xFFFFFFFFu) {  // Mesh::kNoTextureIndex: the barycentric weights
+      o[9] = a;
+      o[10] = b;
+      o[11] = c;
+    } else {
+      const float* x0 = tex + 3 * (size_t)t0;
+      const float* x1 = tex + 3 * (size_t)tidx[3 * (size_t)id + 1];
+      const float* x2 = tex + 3 * (size_t)tidx[3 * (size_t)id + 2];
+      for (int j = 0; j < 3; ++j) o[9 + j] = (x1[j] * a + x2[j] * b) + x0[j] * c;
+    }
+  }
+}
+
 // partitionObjects (DynBVH.cc:1369-1396) alone: slots 0..n-1 hold objects 0..n-1 whose
 // centroid on axis 0 is keys[i]; perm receives the object order after the partition.
 // Returns the split (first slot of the right side).

@@ -38,6 +38,9 @@ def load() -> C.CDLL:
         lib.oracle_build.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
         lib.oracle_build_approx.restype = C.c_int
         lib.oracle_build_approx.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
+        lib.oracle_shade.restype = None
+        lib.oracle_shade.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P, P, P, P, P, P, P,
+                                     P, C.c_int64, P]
         lib.oracle_partition_objects.restype = C.c_int
         lib.oracle_partition_objects.argtypes = [P, C.c_int, C.c_float, P]
         lib.oracle_rtsah.restype = None
@@ -96,6 +99,26 @@ def build_bvh_approx(verts, idx, rtsah: bool = True):
     if used < 0:
         raise ValueError("oracle build failed")
     return nodes, ids, used
+
+
+def shade(verts, idx, rays, hits, prim_kind=0, normals=None, normal_idx=None, tex=None,
+          tex_idx=None):
+    """Hit position, shading normal, geometric normal and texture coordinates per ray
+    (12 floats; zeros for misses)."""
+    v, i = _mesh(verts, idx)
+    arr = [np.ascontiguousarray(rays[k], np.float32) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
+    ptrs = (C.c_void_p * 6)(*[a.ctypes.data for a in arr])
+    prim = np.ascontiguousarray(hits["prim"], np.int32)
+    t = np.ascontiguousarray(hits["t"], np.float32)
+    u = np.ascontiguousarray(hits["u"], np.float32)
+    w = np.ascontiguousarray(hits["v"], np.float32)
+    keep = [np.ascontiguousarray(x, np.float32 if k % 2 == 0 else np.uint32) if x is not None else None
+            for k, x in enumerate((normals, normal_idx, tex, tex_idx))]
+    ps = [None if x is None else _p(x) for x in keep]
+    out = np.zeros((len(prim), 12), np.float32)
+    load().oracle_shade(_p(v), len(v), _p(i), len(i), int(prim_kind), ps[0], ps[1], ps[2], ps[3],
+                        C.cast(ptrs, C.c_void_p), _p(prim), _p(t), _p(u), _p(w), len(prim), _p(out))
+    return out
 
 
 def partition_objects(keys, position):

@@ -82,6 +82,11 @@ class Camera(C.Structure):
                 ("yres", C.c_int), ("spp", C.c_int), ("seed", C.c_uint64)]
 
 
+class Shading(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("px", "py", "pz", "nx", "ny", "nz", "gx", "gy", "gz",
+                                         "tu", "tv", "tw")]
+
+
 class BuildOpts(C.Structure):
     _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int),
                 ("approximate", C.c_int)]
@@ -123,6 +128,9 @@ _SIGS = {
     "mh_scene_create": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_void_p)]),
     "mh_scene_destroy": (C.c_int, [_P]),
     "mh_scene_build": (C.c_int, [_P, C.POINTER(BuildOpts)]),
+    "mh_scene_set_attributes": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, _P]),
+    "mh_shade": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64,
+                           C.POINTER(Shading), _P]),
     "mh_dynbvh_build_host": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(BuildOpts),
                                        _P, _P, C.POINTER(C.c_uint32)]),
     "mh_scene_load_dynbvh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, C.c_int]),

@@ -178,6 +178,36 @@ class Scene:
         L.check(L.load().mh_scene_build(self.h, C.byref(o)))
         return self
 
+    # Mesh::vertexNormals/normal_indices, Mesh::texCoords/texture_indices
+    def set_attributes(self, normals=None, normal_idx=None, texcoords=None, tex_idx=None):
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None, 0
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return a.ctypes.data, (len(a) if a.ndim > 1 else len(a) // 3)
+
+        n, nn = arr(normals, np.float32)
+        ni, _ = arr(normal_idx, np.uint32)
+        t, nt = arr(texcoords, np.float32)
+        ti, _ = arr(tex_idx, np.uint32)
+        L.check(L.load().mh_scene_set_attributes(self.h, n, nn, ni, t, nt, ti))
+
+    def shade(self, rays: "DeviceRays", hits: "DeviceHits", stream=None) -> dict:
+        """Hit positions, shading and geometric normals and texture coordinates (host copies)."""
+        keys = ("px", "py", "pz", "nx", "ny", "nz", "gx", "gy", "gz", "tu", "tv", "tw")
+        bufs = {k: DeviceArray(4 * rays.n) for k in keys}
+        o = L.Shading(*[bufs[k].ptr for k in keys])
+        r, h = rays.struct(), hits.struct()
+        L.check(L.load().mh_shade(self.h, C.byref(r), C.byref(h), rays.n, C.byref(o), stream))
+        sync()
+        out = {k: bufs[k].download(np.float32, rays.n) for k in keys}
+        for b in bufs.values():
+            b.free()
+        return out
+
     # AccelerationStructure::buildFromFile (from memory)
     def load_dynbvh(self, ids: np.ndarray, nodes: np.ndarray,
                     prim_kind: int = L.MH_PRIM_KENSLER_SHIRLEY) -> "Scene":

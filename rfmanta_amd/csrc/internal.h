@@ -91,6 +91,7 @@ struct mh_scene {
   // centroids, depth work arrays, the depth histogram and counters
   DevBuf costs, ssize, parent, obox, ocen, anc0, dep0, anc1, dep1, hist, counters;
   DevBuf logtab;  // RTSAH table logarithm (device build)
+  DevBuf normals, normal_idx, texcoords, tex_idx;  // shading attributes (mh_shade)
   std::vector<int64_t> level_off;  // inner records of depth d: levels[level_off[d], level_off[d+1])
 
   std::mutex spill_mu;  // guards the per-stream spill areas
@@ -104,7 +105,8 @@ struct mh_scene {
     return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +
            d_base.bytes + d_idx.bytes + leaves.bytes + levels.bytes + costs.bytes + ssize.bytes +
            parent.bytes + obox.bytes + ocen.bytes + anc0.bytes + dep0.bytes + anc1.bytes +
-           dep1.bytes + hist.bytes + counters.bytes + logtab.bytes;
+           dep1.bytes + hist.bytes + counters.bytes + logtab.bytes + normals.bytes +
+           normal_idx.bytes + texcoords.bytes + tex_idx.bytes;
   }
   // Traversal stack entries a single ray can hold at most (one push per level).
   uint32_t stack_bound(int /*layout*/) const {

@@ -876,6 +876,89 @@ int mh_scene_update(mh_scene* s, void* stream) {
   return MH_OK;
 }
 
+int mh_scene_set_attributes(mh_scene* s, const float* normals, uint32_t n_normals,
+                            const uint32_t* normal_idx, const float* texcoords,
+                            uint32_t n_texcoords, const uint32_t* tex_idx) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  if (!normals != !normal_idx || !texcoords != !tex_idx)
+    return fail(MH_E_INVALID, "an attribute needs both its values and its indices");
+  const size_t n3 = 3 * (size_t)s->nt;
+  if (normals) {
+    for (size_t k = 0; k < n3; ++k)
+      if (normal_idx[k] >= n_normals)
+        return fail(MH_E_INVALID, "normal index %zu = %u out of range", k, normal_idx[k]);
+  }
+  if (texcoords) {
+    for (size_t t = 0; t < s->nt; ++t) {
+      if (tex_idx[3 * t] == MH_NO_TEXTURE_INDEX) continue;
+      for (int k = 0; k < 3; ++k)
+        if (tex_idx[3 * t + k] >= n_texcoords)
+          return fail(MH_E_INVALID, "texture index %zu = %u out of range", 3 * t + k,
+                      tex_idx[3 * t + k]);
+    }
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  s->normals.release();
+  s->normal_idx.release();
+  s->texcoords.release();
+  s->tex_idx.release();
+  if (normals) {
+    if ((rc = s->normals.reserve(std::max<size_t>(1, n_normals) * 12)) ||
+        (rc = s->normal_idx.reserve(n3 * 4)))
+      return rc;
+    HIP_OK(hipMemcpy(s->normals.p, normals, (size_t)n_normals * 12, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->normal_idx.p, normal_idx, n3 * 4, hipMemcpyHostToDevice));
+  }
+  if (texcoords) {
+    if ((rc = s->texcoords.reserve(std::max<size_t>(1, n_texcoords) * 12)) ||
+        (rc = s->tex_idx.reserve(n3 * 4)))
+      return rc;
+    HIP_OK(hipMemcpy(s->texcoords.p, texcoords, (size_t)n_texcoords * 12,
+                     hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->tex_idx.p, tex_idx, n3 * 4, hipMemcpyHostToDevice));
+  }
+  return MH_OK;
+}
+
+int mh_shade(const mh_scene* s, const mh_rays* rays, const mh_hits* hits, int64_t n,
+             const mh_shading* out, void* stream) {
+  if (!s || !rays || !hits || !out) return fail(MH_E_INVALID, "null argument");
+  if (n < 0) return fail(MH_E_INVALID, "negative ray count");
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+    return fail(MH_E_INVALID, "ray origin and direction arrays are required");
+  if (!hits->prim || !hits->t || !hits->u || !hits->v)
+    return fail(MH_E_INVALID, "hits need prim, t, u and v");
+  if (n == 0) return MH_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  mh::DevShade d{};
+  d.rays.ox = rays->ox;
+  d.rays.oy = rays->oy;
+  d.rays.oz = rays->oz;
+  d.rays.dx = rays->dx;
+  d.rays.dy = rays->dy;
+  d.rays.dz = rays->dz;
+  d.prim = hits->prim;
+  d.t = hits->t;
+  d.u = hits->u;
+  d.v = hits->v;
+  d.verts = s->d_verts.as<float>();
+  d.idx = s->d_idx.as<uint32_t>();
+  d.normals = s->normals.as<float>();
+  d.normal_idx = s->normal_idx.as<uint32_t>();
+  d.tex = s->texcoords.as<float>();
+  d.tex_idx = s->tex_idx.as<uint32_t>();
+  float* o[12] = {out->px, out->py, out->pz, out->nx, out->ny, out->nz,
+                  out->gx, out->gy, out->gz, out->tu, out->tv, out->tw};
+  for (int k = 0; k < 12; ++k) d.out[k] = o[k];
+  d.n = n;
+  d.n_tris = (int32_t)s->nt;
+  d.prim_kind = s->prim_kind;
+  HIP_OK((hipError_t)mh::launch_shade(d, stream));
+  return MH_OK;
+}
+
 int mh_trace(const mh_scene* s, const mh_rays* rays, const mh_hits* hits, int64_t n,
              uint32_t flags, const mh_trace_opts* opts, void* stream) {
   return trace_impl(s, rays, hits, n, flags, opts, stream);

@@ -283,4 +283,22 @@ int build_launch_serial(const DevBuild& d, int64_t n_small, int64_t n_big, int32
 int build_launch_ranks(const DevFinish& f, const int64_t* level_off, int n_levels,
                        int64_t n_leaves, void* stream);
 
+// shade.hip: hit positions, normals and texture coordinates of traced rays.
+struct DevShade {
+  DevRays rays;
+  const int32_t* prim;
+  const float *t, *u, *v;
+  const float* verts;
+  const uint32_t* idx;
+  const float* normals;        // null: face normals
+  const uint32_t* normal_idx;
+  const float* tex;            // null: barycentric weights
+  const uint32_t* tex_idx;
+  float* out[12];              // px py pz, nx ny nz, gx gy gz, tu tv tw (each may be null)
+  int64_t n;
+  int32_t n_tris;
+  int32_t prim_kind;
+};
+int launch_shade(const DevShade& s, void* stream);
+
 }  // namespace mh

@@ -1,0 +1,43 @@
+"""mh_shade (hit positions, normals, texture coordinates on the device) against the
+oracle's restatement, bit for bit, on the hits of a device trace."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from rfmanta_amd import _lib as L  # noqa: E402
+from rfmanta_amd import api, scenes  # noqa: E402
+
+KEYS = ("px", "py", "pz", "nx", "ny", "nz", "gx", "gy", "gz", "tu", "tv", "tw")
+
+
+@pytest.mark.parametrize("kind", [L.MH_PRIM_KENSLER_SHIRLEY, L.MH_PRIM_WALD])
+@pytest.mark.parametrize("attrs", ["none", "normals", "texcoords", "both"])
+def test_shade_matches_oracle(oracle, kind, attrs):
+    print(f"shade kind={kind} attrs={attrs}", flush=True)
+    v, i = scenes.config0_scene()
+    rays = scenes.config0_camera(128, 128)
+    rng = np.random.default_rng(3)
+    normals = nidx = tex = tidx = None
+    if attrs in ("normals", "both"):
+        normals = rng.normal(size=(len(v) + 7, 3)).astype(np.float32)
+        nidx = rng.integers(0, len(normals), size=i.shape).astype(np.uint32)
+    if attrs in ("texcoords", "both"):
+        tex = rng.random((len(v) // 2, 3)).astype(np.float32)
+        tidx = rng.integers(0, len(tex), size=i.shape).astype(np.uint32)
+        tidx[::5, 0] = 0xFFFFFFFF  # Mesh::kNoTextureIndex: these triangles have none
+    with api.Scene(v, i) as s:
+        s.build(prim_kind=kind)
+        s.set_attributes(normals, nidx, tex, tidx)
+        dr = api.DeviceRays.upload(rays)
+        dh = api.DeviceHits.alloc(dr.n)
+        s.trace(dr, dh)
+        api.sync()
+        h = dh.download()
+        g = s.shade(dr, dh)
+    assert (h["prim"] >= 0).sum() > 1000
+    o = oracle.shade(v, i, rays, h, prim_kind=kind, normals=normals, normal_idx=nidx, tex=tex,
+                     tex_idx=tidx)
+    for k, key in enumerate(KEYS):
+        assert np.array_equal(g[key].view(np.uint32), o[:, k].view(np.uint32)), key
