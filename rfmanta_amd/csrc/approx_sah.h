@@ -230,11 +230,29 @@ __device__ inline Partition partition_approx(const DevUpdate& u, int node, int b
   return p;
 }
 
+// Record pairs for new inner nodes: from the shared counter (leaf splits of the update,
+// which are rare), or from a range reserved beforehand (one lane building a whole
+// subtree), which keeps the lanes of the device build off one hot counter.
+struct CounterPairs {
+  __device__ unsigned take(const DevUpdate& u) { return atomicAdd(u.next_free, 2u); }
+};
+struct RangePairs {
+  unsigned next, end;
+  __device__ unsigned take(const DevUpdate&) {
+    if (next + 2 > end) return 0xFFFFFFFEu;  // cannot happen: a subtree of n objects needs <= 2(n-1)
+    const unsigned s = next;
+    next += 2;
+    return s;
+  }
+};
+
 // Turns `node` into an inner node over a new record pair whose records wait (axis =
-// kPending) to be built over [b, split) and [split, e). False if the record array is full.
-__device__ inline bool make_inner(const DevUpdate& u, int node, const Partition& p, int b, int e) {
-  const unsigned spot = atomicAdd(u.next_free, 2u);
-  if (spot + 1 >= (unsigned)u.n_records) {
+// kPending) to be built over [b, split) and [split, e). False if no pair is left.
+template <class Pairs>
+__device__ inline bool make_inner(const DevUpdate& u, Pairs& pairs, int node, const Partition& p,
+                                  int b, int e) {
+  const unsigned spot = pairs.take(u);
+  if (spot >= (unsigned)u.n_records || spot + 1 >= (unsigned)u.n_records) {
     atomicExch(u.err, 1);
     return false;
   }
@@ -261,7 +279,8 @@ __device__ inline bool make_inner(const DevUpdate& u, int node, const Partition&
 // every pending record of the subtree under `top`, starting at the pending record
 // `start` (start is top itself, or top's left child), depth-first without a stack: parent
 // links lead back up, and the climb stops at `top`, whose subtree size it then sets.
-__device__ inline void build_loop(const DevUpdate& u, int start, int top) {
+template <class Pairs>
+__device__ inline void build_loop(const DevUpdate& u, Pairs& pairs, int start, int top) {
   RefNode* refs = u.refs;
   int cur = start;
   for (;;) {
@@ -269,7 +288,7 @@ __device__ inline void build_loop(const DevUpdate& u, int start, int top) {
     const int b = refs[cur].child, e = b + refs[cur].children;
     const Partition p = partition_approx(u, cur, b, e);
     if (p.axis != -1) {
-      if (!make_inner(u, cur, p, b, e)) return;  // record array full: reported through err
+      if (!make_inner(u, pairs, cur, p, b, e)) return;  // no pair left: reported through err
       cur = refs[cur].child;
       continue;
     }
@@ -310,8 +329,9 @@ __device__ inline void split_leaf(const DevUpdate& u, int leaf) {
   const int b0 = refs[leaf].child, e0 = b0 + refs[leaf].children;
   const Partition p0 = partition_approx(u, leaf, b0, e0);
   if (p0.axis == -1) return;
-  if (!make_inner(u, leaf, p0, b0, e0)) return;
-  build_loop(u, refs[leaf].child, leaf);
+  CounterPairs pairs;
+  if (!make_inner(u, pairs, leaf, p0, b0, e0)) return;
+  build_loop(u, pairs, refs[leaf].child, leaf);
 }
 
 }  // namespace

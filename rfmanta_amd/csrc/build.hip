@@ -680,15 +680,31 @@ __global__ __launch_bounds__(kBlock) void k_unslot(DevBuild d) {
   d.u.ids[i] = __float_as_int(reinterpret_cast<const float4*>(SL(d) + i)[0].w);
 }
 
-__global__ __launch_bounds__(64) void k_serial(DevBuild d, int64_t n) {
+__global__ __launch_bounds__(64) void k_serial(DevBuild d, int64_t n_small) {
   const int64_t k = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n_small) return;
   const BuildSeg sg = d.small[k];
   RefNode& nd = d.u.refs[sg.rec];
+  const int n = sg.e - sg.b;
   nd.child = sg.b;
-  nd.children = (int16_t)(sg.e - sg.b);
+  nd.children = (int16_t)n;
   nd.axis = kPending;
-  build_loop(d.u, sg.rec, sg.rec);
+  // A subtree of n objects has at most n - 1 inner nodes: reserve their pairs at once.
+  const unsigned need = 2u * (unsigned)(n - 1);
+  const unsigned base = need ? atomicAdd(d.u.next_free, need) : 0u;
+  if (need && base + need > (unsigned)d.u.n_records) {
+    atomicExch(d.u.err, 1);
+    return;
+  }
+  RangePairs pairs{base, base + need};
+  build_loop(d.u, pairs, sg.rec, sg.rec);
+  for (unsigned r = pairs.next; r < base + need; ++r) {  // unused records: marked as holes
+    RefNode& h = d.u.refs[r];
+    h.child = 0;
+    h.children = kHole;
+    h.axis = 0;
+    h.bits = 0;
+  }
 }
 
 // std::sort of a large leaf's ids: bitonic sort of the range padded to a power of two
@@ -751,11 +767,13 @@ __global__ __launch_bounds__(kBlock) void k_rank_level(DevFinish f, int64_t firs
   f.rank[c] = rk + 1;
   f.rank[c + 1] = rk + 1 + f.ic[c];
 }
+// Records in use move to their reference numbers; records from the node count on are
+// left as DynBVH::allocate leaves them after a reset. Holes (records the one-lane
+// subtrees reserved but did not use) have no number and vanish.
 __global__ __launch_bounds__(kBlock) void k_emit(DevFinish f) {
   const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t used = *f.tcount;
   if (x >= f.n_records) return;
-  if (x >= used) {  // unused records as DynBVH::allocate leaves them after a reset
+  if (x >= f.n_final) {
     RefNode b;
     for (int a = 0; a < 3; ++a) {
       b.box.lo[a] = FLT_MAX;
@@ -766,10 +784,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(DevFinish f) {
     b.bits = 0;
     b.children = 0;
     f.refs[x] = b;
-    return;
   }
+  if (x >= (int64_t)*f.tcount) return;
   const int32_t to = f.fin[x];
-  if (to < 0 || to >= f.n_records) return;  // every record in use is numbered; a guard only
+  if (to < 0 || to >= f.n_final) return;  // a hole
   RefNode t = f.trefs[x];
   t.bits = 0;
   if (t.children == 0) t.child = 2 * f.rank[x] + 1;

@@ -548,17 +548,20 @@ int build_device(mh_scene* s, int rtsah) {
   f.ssize = s->ssize.as<uint32_t>();
   f.rt = rt.as<float>();
   f.logtab = s->logtab.as<float>();
+  const uint32_t n_final = (uint32_t)(level_off.back() + n_leaves);
   f.n_records = (int32_t)nrec;
+  f.n_final = (int32_t)n_final;
   f.large = (uint32_t)(nrec / 20);
   f.rtsah = rtsah;
   const int n_levels = (int)level_off.size() - 1;
   HIP_OK((hipError_t)mh::build_launch_ranks(f, level_off.data(), n_levels, n_leaves, st));
+  HIP_OK(hipMemcpyAsync(cnt, &n_final, 4, hipMemcpyHostToDevice, st));
   mh::DevUpdate u = d.u;
   u.refs = s->refs.as<mh::RefNode>();
   u.pairs = s->pairs.as<mh::PairNode>();
   u.big_count = s->big.as<int32_t>();
   HIP_OK((hipError_t)mh::launch_rebuild_pairs(u, st));
-  const bool root_leaf = used[0] == 1;
+  const bool root_leaf = n_final == 1;
   if (root_leaf && T >= (uint32_t)mh::kLeafCountInline) {
     const int32_t n = (int32_t)T;
     HIP_OK(hipMemcpyAsync(s->big.p, &n, 4, hipMemcpyHostToDevice, st));
@@ -573,7 +576,7 @@ int build_device(mh_scene* s, int rtsah) {
   HIP_OK((hipError_t)mh::launch_prim_setup(r, st));
   HIP_OK(hipStreamSynchronize(st));
   s->root_ref = root_leaf ? ~(int)std::min<uint32_t>(T, mh::kLeafCountInline) : 0;
-  s->next_free = used[0];
+  s->next_free = n_final;
   s->n_records = (uint32_t)nrec;
   s->n_inner = (uint32_t)level_off.back();
   s->n_leaves = (uint32_t)n_leaves;

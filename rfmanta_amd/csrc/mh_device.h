@@ -205,6 +205,7 @@ int launch_copy(void* dst, const void* src, size_t bytes, int cu_count, void* st
 
 
 // Device build (build.hip): DynBVH::rebuild with the approximate SAH at every node.
+constexpr int16_t kHole = -1;     // children of a temporary record no node uses
 constexpr int kBuildSerial = 128;  // nodes of at most this many objects: one lane each
 constexpr int kBuildVB = 2048;     // slots per virtual block of the level passes
 constexpr int kBuildBins = 16;     // BVH_num_samples (DynBVH.cc:34)
@@ -272,6 +273,7 @@ struct DevFinish {
   float* rt;               // RTSAH pairs (c0, c1) per final record
   const float* logtab;
   int32_t n_records;
+  int32_t n_final;         // records of the tree (inner + leaves)
   uint32_t large;
   int rtsah;
 };

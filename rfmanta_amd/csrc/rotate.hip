@@ -315,6 +315,11 @@ __global__ __launch_bounds__(kBlock) void k_parents(DevUpdate u, int32_t* anc, i
     dep[0] = 0;
   }
   const RefNode nd = u.refs[r];
+  if (nd.children == kHole) {  // a record of the device build that no node uses
+    anc[r] = -1;
+    dep[r] = 0;
+    return;
+  }
   if (nd.children != 0) return;
   anc[nd.child] = (int32_t)r;
   anc[nd.child + 1] = (int32_t)r;
@@ -339,30 +344,58 @@ __global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, int3
 }
 
 // hist[d] = inner records at depth d (d < kMaxDepth), hist[kMaxDepth] = leaves,
-// hist[kMaxDepth + 1] = deepest depth.
+// hist[kMaxDepth + 1] = deepest depth. Counted in LDS first: one global atomic per bin
+// and workgroup instead of one per record on a few hot addresses.
 __global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* dep,
                                                        unsigned* hist) {
+  __shared__ unsigned lh[kMaxDepth + 2];
+  for (int i = threadIdx.x; i < kMaxDepth + 2; i += kBlock) lh[i] = 0;
+  __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= used(u.next_free, u.n_records)) return;
-  const int d = dep[r];
-  atomicMax(hist + kMaxDepth + 1, (unsigned)d);
-  if (u.refs[r].children != 0)
-    atomicAdd(hist + kMaxDepth, 1u);
-  else if (d < kMaxDepth)
-    atomicAdd(hist + d, 1u);
+  if (r < used(u.next_free, u.n_records) && u.refs[r].children != kHole) {
+    const int d = dep[r];
+    atomicMax(&lh[kMaxDepth + 1], (unsigned)d);
+    if (u.refs[r].children != 0)
+      atomicAdd(&lh[kMaxDepth], 1u);
+    else if (d < kMaxDepth)
+      atomicAdd(&lh[d], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMaxDepth + 2; i += kBlock) {
+    if (!lh[i]) continue;
+    if (i == kMaxDepth + 1)
+      atomicMax(hist + i, lh[i]);
+    else
+      atomicAdd(hist + i, lh[i]);
+  }
 }
 
 // cursor[d]: next position for depth d in levels; cursor[kMaxDepth]: next leaf position.
+// Each workgroup reserves one range per bin, then its records take places inside it
+// (the order within a depth does not matter). Inner depths are < kMaxDepth (checked by
+// the host before this pass).
 __global__ __launch_bounds__(kBlock) void k_depth_scatter(DevUpdate u, const int32_t* dep,
                                                           unsigned* cursor, int32_t* levels,
                                                           int32_t* leaves) {
+  __shared__ unsigned cnt[kMaxDepth + 1], base[kMaxDepth + 1];
+  for (int i = threadIdx.x; i <= kMaxDepth; i += kBlock) cnt[i] = 0;
+  __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= used(u.next_free, u.n_records)) return;
-  if (u.refs[r].children != 0) {
-    leaves[atomicAdd(cursor + kMaxDepth, 1u)] = (int32_t)r;
-  } else {
-    levels[atomicAdd(cursor + dep[r], 1u)] = (int32_t)r;
+  int bin = -1;
+  unsigned local = 0;
+  if (r < used(u.next_free, u.n_records) && u.refs[r].children != kHole) {
+    const int d = dep[r];
+    bin = u.refs[r].children != 0 ? kMaxDepth : (d < kMaxDepth ? d : -1);
+    if (bin >= 0) local = atomicAdd(&cnt[bin], 1u);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i <= kMaxDepth; i += kBlock)
+    if (cnt[i]) base[i] = atomicAdd(cursor + i, cnt[i]);
+  __syncthreads();
+  if (bin == kMaxDepth)
+    leaves[base[bin] + local] = (int32_t)r;
+  else if (bin >= 0)
+    levels[base[bin] + local] = (int32_t)r;
 }
 
 unsigned grid(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
