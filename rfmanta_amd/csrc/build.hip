@@ -101,18 +101,8 @@ __device__ __forceinline__ float slot_c(const Slot* s, int64_t i, int a) {
   return reinterpret_cast<const float*>(s + i)[a];
 }
 
-// The level node that owns virtual block v (vb0 ascending).
-__device__ __forceinline__ int find_seg(const DevBuild& d, int64_t v) {
-  int lo = 0, hi = d.nseg - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (d.seg[mid].vb0 <= v)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
+// The level node that owns virtual block v.
+__device__ __forceinline__ int find_seg(const DevBuild& d, int64_t v) { return d.vbseg[v]; }
 
 struct VRange {
   int s;
@@ -235,6 +225,15 @@ __global__ __launch_bounds__(1024) void k_scan_segments(DevBuild d) {
     __syncthreads();
   }
   if (threadIdx.x == 0) d.ctr[kCtrNvb] = carry;
+}
+
+// vbseg[v] = the node of virtual block v, for every virtual block of the level.
+__global__ __launch_bounds__(kBlock) void k_vb_map(DevBuild d) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= d.nseg) return;
+  const BuildSeg sg = d.seg[s];
+  const int nvb = (sg.e - sg.b + kVB - 1) / kVB;
+  for (int k = 0; k < nvb; ++k) d.vbseg[sg.vb0 + k] = s;
 }
 
 __global__ __launch_bounds__(kBlock) void k_cbounds(DevBuild d) {
@@ -869,6 +868,7 @@ int build_launch_level(const DevBuild& d, int64_t nvb_bound, void* stream) {
   hipLaunchKernelGGL(k_acc_clear, dim3(grid((int64_t)(kKeyFields + kCntFields) * d.nseg)),
                      dim3(kBlock), 0, s, d);
   hipLaunchKernelGGL(k_scan_segments, dim3(1), dim3(1024), 0, s, d);
+  hipLaunchKernelGGL(k_vb_map, dim3(gs), dim3(kBlock), 0, s, d);
   hipLaunchKernelGGL(k_cbounds, dim3(gv), dim3(kBlock), 0, s, d);
   hipLaunchKernelGGL(k_seg_axis, dim3(gs), dim3(kBlock), 0, s, d);
   hipLaunchKernelGGL(k_bins, dim3(gv), dim3(kBlock), 0, s, d);

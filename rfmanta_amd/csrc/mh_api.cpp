@@ -428,14 +428,14 @@ int build_device(mh_scene* s, int rtsah) {
     HIP_OK(hipMemcpy(s->logtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
   }
   // Work arrays of this build only.
-  DevBuf slots, trefs, seg0, seg1, small, big, plan, acck, accc, ctr, vcnt, gscan, scratch,
-      rootk, ic, rank, fin, rt;
+  DevBuf slots, trefs, seg0, seg1, small, big, plan, acck, accc, ctr, vbseg, vcnt, gscan,
+      scratch, rootk, ic, rank, fin, rt;
   if ((rc = slots.reserve((size_t)T * 48)) || (rc = trefs.reserve(nrec * 32)) ||
       (rc = seg0.reserve(cap * 16)) || (rc = seg1.reserve(cap * 16)) ||
       (rc = small.reserve((size_t)T * 16)) || (rc = big.reserve(cap * 16)) ||
       (rc = plan.reserve(cap * 32)) || (rc = acck.reserve(cap * mh::kKeyFields * 8)) ||
       (rc = accc.reserve(cap * mh::kCntFields * 4)) || (rc = ctr.reserve(16)) ||
-      (rc = vcnt.reserve(vcap * 8)) || (rc = gscan.reserve(2 * (vcap + 1) * 4)) ||
+      (rc = vbseg.reserve(vcap * 4)) || (rc = vcnt.reserve(vcap * 8)) || (rc = gscan.reserve(2 * (vcap + 1) * 4)) ||
       (rc = scratch.reserve(nrec * 4)) || (rc = rootk.reserve(48)) ||
       (rc = ic.reserve(nrec * 4)) || (rc = rank.reserve(nrec * 4)) ||
       (rc = fin.reserve(nrec * 4)) || (rc = rt.reserve(nrec * 8)))
@@ -472,6 +472,7 @@ int build_device(mh_scene* s, int rtsah) {
   d.accc = accc.as<unsigned>();
   d.cap = cap;
   d.ctr = ctr.as<unsigned>();
+  d.vbseg = vbseg.as<int32_t>();
   d.vcnt = vcnt.as<unsigned>();
   d.gscan = gscan.as<unsigned>();
   d.vcap = vcap;

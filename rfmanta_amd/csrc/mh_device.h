@@ -206,7 +206,7 @@ int launch_copy(void* dst, const void* src, size_t bytes, int cu_count, void* st
 
 // Device build (build.hip): DynBVH::rebuild with the approximate SAH at every node.
 constexpr int16_t kHole = -1;     // children of a temporary record no node uses
-constexpr int kBuildSerial = 128;  // nodes of at most this many objects: one lane each
+constexpr int kBuildSerial = 64;  // nodes of at most this many objects: one lane each
 constexpr int kBuildVB = 2048;     // slots per virtual block of the level passes
 constexpr int kBuildBins = 16;     // BVH_num_samples (DynBVH.cc:34)
 
@@ -249,6 +249,7 @@ struct DevBuild {
   unsigned* accc;                 // [kCntFields][cap]
   int64_t cap;                    // capacity of the node lists
   unsigned* ctr;                  // kCtr*
+  int32_t* vbseg;                 // per virtual block: its node
   unsigned* vcnt;                 // per virtual block: left, right misplaced counts
   unsigned* gscan;                // exclusive scans of vcnt (left at [0, nvb], right after)
   int64_t vcap;                   // capacity of the virtual block arrays

@@ -343,57 +343,89 @@ __global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, int3
   dep_out[r] = dep_in[r] + dep_in[a];
 }
 
-// hist[d] = inner records at depth d (d < kMaxDepth), hist[kMaxDepth] = leaves,
-// hist[kMaxDepth + 1] = deepest depth. Counted in LDS first: one global atomic per bin
-// and workgroup instead of one per record on a few hot addresses.
+// hist[d] = inner records at depth d (d < kMaxDepth), counted in LDS first (one global
+// atomic per bin and workgroup). hist[kMaxDepth] is set when an inner record lies at
+// kMaxDepth or deeper. Holes are skipped.
 __global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* dep,
                                                        unsigned* hist) {
-  __shared__ unsigned lh[kMaxDepth + 2];
-  for (int i = threadIdx.x; i < kMaxDepth + 2; i += kBlock) lh[i] = 0;
+  __shared__ unsigned lh[kMaxDepth];
+  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock) lh[i] = 0;
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r < used(u.next_free, u.n_records) && u.refs[r].children != kHole) {
+  if (r < used(u.next_free, u.n_records) && u.refs[r].children == 0) {
     const int d = dep[r];
-    atomicMax(&lh[kMaxDepth + 1], (unsigned)d);
-    if (u.refs[r].children != 0)
-      atomicAdd(&lh[kMaxDepth], 1u);
-    else if (d < kMaxDepth)
+    if (d < kMaxDepth)
       atomicAdd(&lh[d], 1u);
+    else
+      hist[kMaxDepth] = 1u;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kMaxDepth + 2; i += kBlock) {
-    if (!lh[i]) continue;
-    if (i == kMaxDepth + 1)
-      atomicMax(hist + i, lh[i]);
-    else
-      atomicAdd(hist + i, lh[i]);
+  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock)
+    if (lh[i]) atomicAdd(hist + i, lh[i]);
+}
+
+// A full binary tree has one leaf more than inner records, and its deepest record is a
+// leaf one below the deepest inner record: hist[kMaxDepth] = leaves, hist[kMaxDepth + 1] =
+// deepest depth (kMaxDepth when the tree is too deep). One workgroup.
+__global__ __launch_bounds__(kBlock) void k_depth_finish(unsigned* hist) {
+  __shared__ unsigned inner[kBlock], deep[kBlock];
+  unsigned n = 0, dmax = 0;
+  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock) {
+    n += hist[i];
+    if (hist[i]) dmax = (unsigned)i + 1 > dmax ? (unsigned)i + 1 : dmax;
+  }
+  inner[threadIdx.x] = n;
+  deep[threadIdx.x] = dmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tn = 0, td = 0;
+    for (int k = 0; k < kBlock; ++k) {
+      tn += inner[k];
+      td = deep[k] > td ? deep[k] : td;
+    }
+    const bool too_deep = hist[kMaxDepth] != 0;
+    hist[kMaxDepth] = tn + 1;
+    hist[kMaxDepth + 1] = too_deep ? (unsigned)kMaxDepth : td;
   }
 }
 
 // cursor[d]: next position for depth d in levels; cursor[kMaxDepth]: next leaf position.
-// Each workgroup reserves one range per bin, then its records take places inside it
-// (the order within a depth does not matter). Inner depths are < kMaxDepth (checked by
-// the host before this pass).
+// Each workgroup reserves one range per depth and one for its leaves, then its records
+// take places inside them (order within a list does not matter). Inner depths are
+// < kMaxDepth (checked by the host before this pass).
 __global__ __launch_bounds__(kBlock) void k_depth_scatter(DevUpdate u, const int32_t* dep,
                                                           unsigned* cursor, int32_t* levels,
                                                           int32_t* leaves) {
-  __shared__ unsigned cnt[kMaxDepth + 1], base[kMaxDepth + 1];
-  for (int i = threadIdx.x; i <= kMaxDepth; i += kBlock) cnt[i] = 0;
-  __syncthreads();
+  __shared__ unsigned cnt[kMaxDepth], base[kMaxDepth], scan[kBlock];
+  __shared__ unsigned leaf_base;
+  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock) cnt[i] = 0;
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  int bin = -1;
-  unsigned local = 0;
-  if (r < used(u.next_free, u.n_records) && u.refs[r].children != kHole) {
-    const int d = dep[r];
-    bin = u.refs[r].children != 0 ? kMaxDepth : (d < kMaxDepth ? d : -1);
-    if (bin >= 0) local = atomicAdd(&cnt[bin], 1u);
+  int bin = -1;  // depth of an inner record, kMaxDepth for a leaf
+  if (r < used(u.next_free, u.n_records)) {
+    const int16_t c = u.refs[r].children;
+    if (c > 0)
+      bin = kMaxDepth;
+    else if (c == 0 && dep[r] < kMaxDepth)
+      bin = dep[r];
   }
+  const unsigned is_leaf = bin == kMaxDepth ? 1u : 0u;
+  scan[threadIdx.x] = is_leaf;
   __syncthreads();
-  for (int i = threadIdx.x; i <= kMaxDepth; i += kBlock)
+  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive scan of the leaf flags
+    const unsigned x = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0u;
+    __syncthreads();
+    scan[threadIdx.x] += x;
+    __syncthreads();
+  }
+  unsigned local = 0;
+  if (bin >= 0 && bin < kMaxDepth) local = atomicAdd(&cnt[bin], 1u);
+  if (threadIdx.x == kBlock - 1) leaf_base = scan[kBlock - 1] ? atomicAdd(cursor + kMaxDepth, scan[kBlock - 1]) : 0u;
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock)
     if (cnt[i]) base[i] = atomicAdd(cursor + i, cnt[i]);
   __syncthreads();
-  if (bin == kMaxDepth)
-    leaves[base[bin] + local] = (int32_t)r;
+  if (is_leaf)
+    leaves[leaf_base + scan[threadIdx.x] - 1] = (int32_t)r;
   else if (bin >= 0)
     levels[base[bin] + local] = (int32_t)r;
 }
@@ -444,6 +476,7 @@ int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc
                        even ? dep0 : dep1, even ? anc1 : anc0, even ? dep1 : dep0);
   }
   hipLaunchKernelGGL(k_depth_hist, dim3(g), dim3(kBlock), 0, s, u, dep0, hist);
+  hipLaunchKernelGGL(k_depth_finish, dim3(1), dim3(kBlock), 0, s, hist);
   return (int)hipGetLastError();
 }
 

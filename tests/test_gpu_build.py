@@ -19,7 +19,7 @@ from rfmanta_amd import api, scenes  # noqa: E402
 
 def _meshes():
     out = []
-    for n in (1, 2, 3, 128, 129, 300):
+    for n in (1, 2, 3, 64, 65, 128, 129, 300):
         out.append((f"soup{n}",) + scenes.soup(n, seed=n))
     out.append(("soup20000",) + scenes.soup(20000, seed=3))
     out.append(("heightfield",) + scenes.heightfield(150, 90, seed=1))
