@@ -259,6 +259,8 @@ def main() -> int:
 
     if "c2" not in skip:
         out["c2_batching"] = bench_c2(api, L, scenes, scene, verts, idx, args)
+        out["c2_incoherent"] = bench_c2(api, L, scenes, scene, verts, idx, args,
+                                        incoherent=True)
     if "c3" not in skip:
         c3 = bench_c3(api, scenes, scene, args)
         out["c3_serving"] = c3
@@ -309,9 +311,19 @@ def bench_build(api, L, verts, idx, host_build_s, sets, args) -> dict:
     return res
 
 
-def bench_c2(api, L, scenes, scene, verts, idx, args) -> dict:
+def bench_c2(api, L, scenes, scene, verts, idx, args, incoherent=False) -> dict:
+    """64 uneven batches against the 1M-triangle tree: 64 plain launches vs one frame.
+    incoherent=True keeps the batch sizes but makes every batch diffuse-bounce rays."""
     t0 = time.time()
     batches = scenes.config2_batches(verts, idx)
+    if incoherent:
+        start = 0
+        inc = []
+        for _, r in batches:
+            n = len(r["ox"])
+            inc.append(("bounce", scenes.batch_rays("bounce", n, verts, idx, 1, start)))
+            start += n
+        batches = inc
     dev = []
     for kind, r in batches:
         dr = api.DeviceRays.upload(r)

@@ -85,21 +85,48 @@ struct Lane {
   bool any;
 };
 
-// One traversal step of traverse.h's loops with a run-time closest/any-hit flag;
-// returns false when the ray is finished.
-template <int PRIM, int LAYOUT, int S, int BLOCK>
-__device__ __forceinline__ bool step(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
+// Per-lane refill schedule, REF layout: one traversal step of traverse.h's trace_ref with
+// a run-time closest/any-hit flag; returns false when the ray is finished.
+template <int PRIM, int S, int BLOCK>
+__device__ __forceinline__ bool step_ref(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
   unsigned long long ntri = 0;
-  if (LAYOUT == kLayoutPair) {
+  const RefNode nd = sc.refs[L.cur];
+  float tn;
+  if (slab(nd.box.lo[0], nd.box.hi[0], nd.box.lo[1], nd.box.hi[1], nd.box.lo[2], nd.box.hi[2],
+           L.r, L.h.minT, tn)) {
+    if (nd.children != 0) {
+      const bool done = L.any ? leaf<true, PRIM, false>(sc, nd.child, nd.children, L.r, L.h, ntri)
+                              : leaf<false, PRIM, false>(sc, nd.child, nd.children, L.r, L.h, ntri);
+      if (done) return false;
+    } else {
+      const int front = L.any ? ((nd.bits & 1) ? 0 : 1)
+                              : (((L.r.dpos >> (nd.axis & 3)) & 1) ? 0 : 1);
+      st.push(nd.child + 1 - front, 0.f);
+      L.cur = nd.child + front;
+      return true;
+    }
+  }
+  if (st.empty()) return false;
+  st.pop(L.cur, tn);
+  return true;
+}
+
+// Per-lane refill schedule, PAIR layout: the walk of traverse.h's trace_pair, resumed
+// from the lane's state and left after each leaf so that the wave can refill lanes whose
+// rays have finished. Returns false when the ray is finished.
+template <bool ANY, int PRIM, int S, int BLOCK, bool FAST>
+__device__ __forceinline__ bool walk_pair(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
+  unsigned long long ntri = 0;
+  for (;;) {
+    bool at_leaf = false;
     if (L.cur >= 0) {
       const float4* np = reinterpret_cast<const float4*>(sc.pairs + L.cur);
       const float4 A = np[0], B = np[1], C = np[2];
       const int4 M = reinterpret_cast<const int4*>(np)[3];
       float tn0, tn1;
-      const bool h0 = slab(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
-      const bool h1 = slab(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
-      const bool one_first =
-          L.any ? ((M.z >> 2) & 1) == 0 : ((L.r.dpos >> (M.z & 3)) & 1) == 0;
+      const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
+      const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
+      const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : ((L.r.dpos >> (M.z & 3)) & 1) == 0;
       const int fref = one_first ? M.y : M.x;
       const int bref = one_first ? M.x : M.y;
       const bool fh = one_first ? h1 : h0;
@@ -107,51 +134,33 @@ __device__ __forceinline__ bool step(const DevScene& sc, Lane& L, LaneStack<S, B
       if (fh) {
         if (bh) st.push(bref, one_first ? tn0 : tn1);
         L.cur = fref;
-        return true;
+        continue;
       }
       if (bh) {
         L.cur = bref;
-        return true;
+        continue;
       }
     } else {
       const int code = ~L.cur;
       const int first = code >> kLeafCountBits;
       int cnt = code & kLeafCountInline;
       if (cnt == kLeafCountInline) cnt = sc.big_count[first];
-      const bool done = L.any ? leaf<true, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)
-                              : leaf<false, PRIM, false>(sc, first, cnt, L.r, L.h, ntri);
-      if (done) return false;
+      if (leaf<ANY, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)) return false;
+      at_leaf = true;
     }
+    bool go = false;
     while (!st.empty()) {
       int ref;
-      float tn;
-      st.pop(ref, tn);
-      if (tn <= L.h.minT) {
+      float t;
+      st.pop(ref, t);
+      if (t <= L.h.minT) {
         L.cur = ref;
-        return true;
+        go = true;
+        break;
       }
     }
-    return false;
-  } else {
-    const RefNode nd = sc.refs[L.cur];
-    float tn;
-    if (slab(nd.box.lo[0], nd.box.hi[0], nd.box.lo[1], nd.box.hi[1], nd.box.lo[2], nd.box.hi[2],
-             L.r, L.h.minT, tn)) {
-      if (nd.children != 0) {
-        const bool done = L.any ? leaf<true, PRIM, false>(sc, nd.child, nd.children, L.r, L.h, ntri)
-                                : leaf<false, PRIM, false>(sc, nd.child, nd.children, L.r, L.h, ntri);
-        if (done) return false;
-      } else {
-        const int front = L.any ? ((nd.bits & 1) ? 0 : 1)
-                                : (((L.r.dpos >> (nd.axis & 3)) & 1) ? 0 : 1);
-        st.push(nd.child + 1 - front, 0.f);
-        L.cur = nd.child + front;
-        return true;
-      }
-    }
-    if (st.empty()) return false;
-    st.pop(L.cur, tn);
-    return true;
+    if (!go) return false;
+    if (at_leaf) return true;
   }
 }
 
@@ -290,8 +299,30 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(DevScene sc,
         }
         if (g >= 0) active = start_ray<PRIM, LAYOUT, S, BLOCK>(sc, root, batches, s_start, nb, g, L, st);
       }
-      if (active) {
-        if (!step<PRIM, LAYOUT, S, BLOCK>(sc, L, st)) {
+      if constexpr (LAYOUT == kLayoutPair) {
+        // min/max slab test when every ray of the wave allows it (wave-uniform choice)
+        const bool fast = __ballot(active && !regular_ray(L.r, L.h.minT)) == 0ull;
+        if (active) {
+          bool more;
+          if (KIND == kKindClosest)
+            more = fast ? walk_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
+                        : walk_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+          else if (KIND == kKindAny)
+            more = fast ? walk_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
+                        : walk_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+          else if (L.any)
+            more = fast ? walk_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
+                        : walk_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+          else
+            more = fast ? walk_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
+                        : walk_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+          if (!more) {
+            finish(batches, sc, L);
+            active = false;
+          }
+        }
+      } else if (active) {
+        if (!step_ref<PRIM, S, BLOCK>(sc, L, st)) {
           finish(batches, sc, L);
           active = false;
         }
