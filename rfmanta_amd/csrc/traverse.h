@@ -108,17 +108,11 @@ __device__ __forceinline__ void trace_pair(const DevScene& sc, const RefNode& ro
       if (STATS) nbox += 2;
       // Front child: isLeftCheaper (any-hit) or the direction's sign on the split axis.
       const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : ((r.dpos >> (M.z & 3)) & 1) == 0;
-      const int fref = one_first ? M.y : M.x;
-      const int bref = one_first ? M.x : M.y;
-      const bool fh = one_first ? h1 : h0;
-      const bool bh = one_first ? h0 : h1;
-      if (fh) {
-        if (bh) st.push(bref, one_first ? tn0 : tn1);
-        cur = fref;
-        continue;
-      }
-      if (bh) {
-        cur = bref;
+      // Both hit: enter the front child, push the other. One hits: enter it. Written on
+      // the hit masks, so the choice stays in scalar mask operations.
+      if (h0 && h1) st.push(one_first ? M.x : M.y, one_first ? tn0 : tn1);
+      if (h0 || h1) {
+        cur = (h1 && (one_first || !h0)) ? M.y : M.x;
         continue;
       }
     } else {
