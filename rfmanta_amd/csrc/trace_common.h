@@ -173,8 +173,9 @@ static __device__ __forceinline__ bool prim_test(const float* tris, int slot, co
   float4 A = p[0], B = p[1], C = p[2];
   // All 48 bytes are fetched before the first rejection test (one round trip) instead of
   // the compiler deferring the edge loads past the distance test.
-  asm volatile("" : "+v"(A.x), "+v"(A.y), "+v"(A.z), "+v"(A.w), "+v"(B.x), "+v"(B.y), "+v"(B.z),
-               "+v"(B.w), "+v"(C.x), "+v"(C.y), "+v"(C.z), "+v"(C.w));
+  // (Input operands only: the values stay in the registers they were loaded into.)
+  asm volatile("" ::"v"(A.x), "v"(A.y), "v"(A.z), "v"(A.w), "v"(B.x), "v"(B.y), "v"(B.z),
+               "v"(B.w), "v"(C.x), "v"(C.y), "v"(C.z), "v"(C.w));
   if (PRIM == 0) return ks_test(A, B, C, r, minT, t, u, v);
   return wald_test(A, B, C, r, minT, t, u, v);
 }
