@@ -141,6 +141,16 @@ int mh_dynbvh_build_host(const float* verts, uint32_t n_verts, const uint32_t* i
                          uint32_t n_tris, const mh_build_opts* opts, int32_t* ids, void* nodes,
                          uint32_t* num_nodes);
 
+/* PLY mesh ingest, as readPlyFile builds a Mesh (Model/Readers/PlyReader.cc:136-269):
+ * vertex x, y, z (any scalar type, stored as float) transformed by the 3x4 row-major
+ * `transform` (AffineTransform::multiply_point; NULL = identity), faces from the
+ * "vertex_indices" list as triangle fans, then Mesh::removeDegenerateTriangles
+ * (Model/Groups/Mesh.cc:315-359). ascii and binary PLY. With verts and idx NULL, only
+ * *n_verts and *n_tris are set; otherwise the arrays (verts_cap vertices, tris_cap
+ * triangles) receive the mesh. Host only. */
+int mh_ply_read(const char* path, const float* transform, float* verts, uint32_t verts_cap,
+                uint32_t* idx, uint32_t tris_cap, uint32_t* n_verts, uint32_t* n_tris);
+
 /* DynBVH::buildFromFile (DynBVH.cc:2442-2472) from memory: n_ids object ids and n_nodes
  * 32-byte BVHNode records in the reference layout (DynBVH.h:37-77). */
 int mh_scene_load_dynbvh(mh_scene* scene, const int32_t* ids, uint32_t n_ids, const void* nodes,

@@ -129,6 +129,8 @@ _SIGS = {
     "mh_scene_destroy": (C.c_int, [_P]),
     "mh_scene_build": (C.c_int, [_P, C.POINTER(BuildOpts)]),
     "mh_scene_set_attributes": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, _P]),
+    "mh_ply_read": (C.c_int, [C.c_char_p, _P, _P, C.c_uint32, _P, C.c_uint32,
+                              C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "mh_shade": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64,
                            C.POINTER(Shading), _P]),
     "mh_dynbvh_build_host": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(BuildOpts),

@@ -495,6 +495,22 @@ def build_dynbvh_host(verts: np.ndarray, idx: np.ndarray, rtsah: bool = True,
     return nodes, ids, used.value
 
 
+def read_ply(path: str, transform=None):
+    """readPlyFile: (verts float32[V, 3], idx uint32[T, 3]) of a PLY file, as Manta builds
+    the Mesh (fan triangulation, degenerate triangles removed). transform: 3x4 or None."""
+    lib = L.load()
+    xf = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(3, 4)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    p = str(path).encode()
+    L.check(lib.mh_ply_read(p, None if xf is None else xf.ctypes.data, None, 0, None, 0,
+                            C.byref(nv), C.byref(nt)))
+    v = np.empty((nv.value, 3), np.float32)
+    i = np.empty((nt.value, 3), np.uint32)
+    L.check(lib.mh_ply_read(p, None if xf is None else xf.ctypes.data, v.ctypes.data, nv.value,
+                            i.ctypes.data, nt.value, C.byref(nv), C.byref(nt)))
+    return v, i
+
+
 def sync(stream=None) -> None:
     L.check(L.load().mh_stream_sync(stream))
 
