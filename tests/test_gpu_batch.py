@@ -142,3 +142,33 @@ def test_autotune_cache(tmp_path, small):
         f.launch()
         api.sync()
         f.close()
+
+
+@pytest.mark.parametrize("cfg", [
+    {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 4, "refill": 64, "chunk": 0},
+    {"layout": 0, "stack_lds": 8, "block": 128, "blocks_per_cu": 2, "refill": 16, "chunk": 64},
+    {"layout": 1, "stack_lds": 16, "block": 64, "blocks_per_cu": 4, "refill": 64, "chunk": 64},
+])
+def test_frame_wald_matches_oracle(oracle, small, cfg):
+    """The Wald-triangle batch kernels (PAIR and REF layouts, all three schedules)."""
+    from rfmanta_amd import _lib as L
+    v, i, batches = small
+    nodes, ids, _ = oracle.build_bvh(v, i)
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes, prim_kind=L.MH_PRIM_WALD)
+        dev = _device(batches)
+        f = api.Frame(s, dev)
+        f.set_config(cfg)
+        f.launch()
+        api.sync()
+        for (k, rays), (dr, dh, any_hit) in zip(batches, dev):
+            g = dh.download()
+            o = oracle.trace(v, i, nodes, ids, rays, any_hit=any_hit, packet_size=1, prim_kind=1)
+            assert np.array_equal(g["prim"], o["prim"]), (k, len(rays["ox"]))
+            assert np.array_equal(_bits(g["t"]), _bits(o["t"])), k
+            if any_hit:
+                assert np.array_equal(g["occluded"], o["occluded"])
+            else:
+                assert np.array_equal(_bits(g["u"]), _bits(o["u"]))
+                assert np.array_equal(_bits(g["v"]), _bits(o["v"]))
+        f.close()

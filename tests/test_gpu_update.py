@@ -116,3 +116,25 @@ def test_update_then_refit_and_frames(oracle):
         o1 = oracle.trace(vd2, i, ref, st.ids, rays, packet_size=1)
         assert np.array_equal(dh.download()["prim"], o1["prim"])
         fr.close()
+
+
+def test_update_wald_records(oracle):
+    """Wald triangle records are rewritten in the new leaf order after rotations."""
+    from rfmanta_amd import _lib as L
+    v, i = scenes.soup(3000, seed=8)
+    nodes, ids, nn = oracle.build_bvh(v, i)
+    st = oracle.RotState(nodes, ids, nn)
+    rays = scenes.random_rays(4096, 5, (-1.3, -1.3, -1.3), (1.3, 1.3, 1.3))
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes, prim_kind=L.MH_PRIM_WALD)
+        for f in range(3):
+            vd = v.copy()
+            vd[:, 1] += (0.4 * np.sin(4 * v[:, 0] + 0.9 * f)).astype(np.float32)
+            st.update(vd, i)
+            s.set_vertices(vd)
+            s.update()
+            api.sync()
+            g = s.trace_host(rays)
+            o = oracle.trace(vd, i, st.nodes, st.ids, rays, packet_size=1, prim_kind=1)
+            assert np.array_equal(g["prim"], o["prim"]), f
+            assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32)), f
