@@ -151,6 +151,35 @@ int mh_dynbvh_build_host(const float* verts, uint32_t n_verts, const uint32_t* i
 int mh_ply_read(const char* path, const float* transform, float* verts, uint32_t verts_cap,
                 uint32_t* idx, uint32_t tris_cap, uint32_t* n_verts, uint32_t* n_tris);
 
+/* OBJ mesh ingest, as ObjGroup builds a Mesh (Model/Groups/ObjGroup.cc:184-316) from the
+ * file glmReadOBJ reads (Model/Readers/glm/glm.cc:575-960, 1599-1670): vertices, normals
+ * and texture coordinates (u, v, 1) in file order; faces as triangle fans, emitted group by
+ * group in glm's list order; per corner a normal index (a face normal is appended for
+ * corners without one when the file has normals) and a texture index
+ * (MH_NO_TEXTURE_INDEX where a corner has none and the file has texture coordinates); per
+ * triangle a material slot, Mesh::face_material: 0 = the caller's default material, k >= 1
+ * = glm material k-1 (material_names[k]; glm material 0 is NO_ASSIGNED_MATERIAL, then the
+ * newmtl entries of the material library); then Mesh::removeDegenerateTriangles
+ * (Model/Groups/Mesh.cc:315-359). Material parameters and textures are not read. The
+ * normal and texture arrays are NULL when the file has none; normals, normal_idx,
+ * texcoords and tex_idx are the arguments mh_scene_set_attributes takes. Host only. */
+typedef struct mh_obj_mesh {
+  uint32_t n_verts, n_tris, n_normals, n_texcoords, n_materials;
+  const float* verts;                 /* [3 n_verts] */
+  const uint32_t* idx;                /* [3 n_tris], 0-based */
+  const float* normals;               /* [3 n_normals] or NULL */
+  const uint32_t* normal_idx;         /* [3 n_tris] or NULL */
+  const float* texcoords;             /* [3 n_texcoords] or NULL */
+  const uint32_t* tex_idx;            /* [3 n_tris] or NULL */
+  const uint32_t* face_material;      /* [n_tris] */
+  const char* const* material_names;  /* [n_materials]; [0] is "" (the default slot) */
+} mh_obj_mesh;
+
+/* Reads an OBJ file (and the material library it names, beside it) into a mesh the
+ * library owns until mh_obj_free. */
+int mh_obj_read(const char* path, mh_obj_mesh** mesh);
+void mh_obj_free(mh_obj_mesh* mesh);
+
 /* DynBVH::buildFromFile (DynBVH.cc:2442-2472) from memory: n_ids object ids and n_nodes
  * 32-byte BVHNode records in the reference layout (DynBVH.h:37-77). */
 int mh_scene_load_dynbvh(mh_scene* scene, const int32_t* ids, uint32_t n_ids, const void* nodes,

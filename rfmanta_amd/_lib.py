@@ -87,6 +87,16 @@ class Shading(C.Structure):
                                          "tu", "tv", "tw")]
 
 
+class ObjMesh(C.Structure):
+    _fields_ = [("n_verts", C.c_uint32), ("n_tris", C.c_uint32), ("n_normals", C.c_uint32),
+                ("n_texcoords", C.c_uint32), ("n_materials", C.c_uint32),
+                ("verts", C.POINTER(C.c_float)), ("idx", C.POINTER(C.c_uint32)),
+                ("normals", C.POINTER(C.c_float)), ("normal_idx", C.POINTER(C.c_uint32)),
+                ("texcoords", C.POINTER(C.c_float)), ("tex_idx", C.POINTER(C.c_uint32)),
+                ("face_material", C.POINTER(C.c_uint32)),
+                ("material_names", C.POINTER(C.c_char_p))]
+
+
 class BuildOpts(C.Structure):
     _fields_ = [("prim_kind", C.c_int), ("rtsah", C.c_int), ("threads", C.c_int),
                 ("approximate", C.c_int)]
@@ -131,6 +141,8 @@ _SIGS = {
     "mh_scene_set_attributes": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_uint32, _P]),
     "mh_ply_read": (C.c_int, [C.c_char_p, _P, _P, C.c_uint32, _P, C.c_uint32,
                               C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mh_obj_read": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(ObjMesh))]),
+    "mh_obj_free": (None, [C.POINTER(ObjMesh)]),
     "mh_shade": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64,
                            C.POINTER(Shading), _P]),
     "mh_dynbvh_build_host": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(BuildOpts),

@@ -511,6 +511,41 @@ def read_ply(path: str, transform=None):
     return v, i
 
 
+def read_obj(path: str) -> dict:
+    """ObjGroup's Mesh of a Wavefront OBJ file: verts float32[V, 3], idx uint32[T, 3],
+    normals float32[N, 3] and normal_idx uint32[T, 3] (None without normals), texcoords
+    float32[M, 3] and tex_idx uint32[T, 3] (None without texture coordinates),
+    face_material uint32[T] (0 = the default material, k = material_names[k])."""
+    lib = L.load()
+    m = C.POINTER(L.ObjMesh)()
+    L.check(lib.mh_obj_read(str(path).encode(), C.byref(m)))
+    try:
+        o = m.contents
+
+        def arr(p, n, cols, dt):
+            shape = (n, cols) if cols > 1 else (n,)
+            if n == 0:
+                return np.empty(shape, dt)
+            return np.ctypeslib.as_array(p, shape=shape).astype(dt, copy=True)
+
+        out = {
+            "verts": arr(o.verts, o.n_verts, 3, np.float32),
+            "idx": arr(o.idx, o.n_tris, 3, np.uint32),
+            "face_material": arr(o.face_material, o.n_tris, 1, np.uint32),
+            "material_names": [o.material_names[k].decode() for k in range(o.n_materials)],
+            "normals": None, "normal_idx": None, "texcoords": None, "tex_idx": None,
+        }
+        if o.normals:
+            out["normals"] = arr(o.normals, o.n_normals, 3, np.float32)
+            out["normal_idx"] = arr(o.normal_idx, o.n_tris, 3, np.uint32)
+        if o.texcoords:
+            out["texcoords"] = arr(o.texcoords, o.n_texcoords, 3, np.float32)
+            out["tex_idx"] = arr(o.tex_idx, o.n_tris, 3, np.uint32)
+        return out
+    finally:
+        lib.mh_obj_free(m)
+
+
 def sync(stream=None) -> None:
     L.check(L.load().mh_stream_sync(stream))
 
