@@ -120,13 +120,15 @@ __device__ __forceinline__ bool walk_pair(const DevScene& sc, Lane& L, LaneStack
   for (;;) {
     bool at_leaf = false;
     if (L.cur >= 0) {
-      const float4* np = reinterpret_cast<const float4*>(sc.pairs + L.cur);
-      const float4 A = np[0], B = np[1], C = np[2];
-      const int4 M = reinterpret_cast<const int4*>(np)[3];
+      const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
+      const float4 A = fetch_record(pt, L.cur, 0), B = fetch_record(pt, L.cur, 16),
+                   C = fetch_record(pt, L.cur, 32), Mf = fetch_record(pt, L.cur, 48);
+      const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
+                               __float_as_int(Mf.w));
       float tn0, tn1;
       const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
       const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
-      const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : ((L.r.dpos >> (M.z & 3)) & 1) == 0;
+      const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : __builtin_amdgcn_ubfe(L.r.dpos, M.z, 1) == 0;
       const int fref = one_first ? M.y : M.x;
       const int bref = one_first ? M.x : M.y;
       const bool fh = one_first ? h1 : h0;

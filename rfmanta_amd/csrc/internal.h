@@ -121,6 +121,8 @@ struct mh_scene {
     d.big_count = big.as<const int32_t>();
     d.root_ref = root_ref;
     d.prim_kind = prim_kind;
+    d.n_pairs = (uint32_t)(pairs.bytes / sizeof(mh::PairNode));
+    d.n_slots = (uint32_t)(tris.bytes / (12 * sizeof(float)));
     return d;
   }
 };

@@ -192,7 +192,7 @@ int upload(mh_scene* s) {
         b[5] = c.box.hi[2];
         P.ref[side] = encode(n.child + side);
       }
-      P.meta = (int32_t)n.axis | ((n.bits & 1) << 2);
+      P.meta = (int32_t)(n.axis & 3) | ((n.bits & 1) << 2);
       P.node = r;
     }
   s->root_ref = encode(0);

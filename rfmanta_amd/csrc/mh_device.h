@@ -29,7 +29,9 @@ namespace mh {
 // ~((first_slot << 4) | c) with c = min(count, 15); c == 15 means "read big_count".
 constexpr int kLeafCountBits = 4;
 constexpr int kLeafCountInline = 15;
-constexpr uint32_t kMaxObjects = 1u << 27;
+// Fewer than 2^26 objects: leaf codes hold first_slot << 4 in an int, and the pair table
+// (64 B per object) stays within the 4 GiB that a buffer descriptor's byte offsets reach.
+constexpr uint32_t kMaxObjects = 1u << 26;
 
 struct alignas(16) PairNode {
   // child 0: lo.x hi.x lo.y hi.y | lo.z hi.z ; child 1: lo.x hi.x | lo.y hi.y lo.z hi.z
@@ -49,6 +51,8 @@ struct DevScene {
   const int32_t* big_count;
   int32_t root_ref;   // pair-encoded reference of record 0
   int32_t prim_kind;  // 0 KS, 1 Wald
+  uint32_t n_pairs;   // PairNodes allocated (the bound of the pair table's descriptor)
+  uint32_t n_slots;   // triangle records allocated (the bound of the triangle table's)
 };
 
 struct DevRays {

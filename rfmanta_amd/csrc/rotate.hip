@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
     b[5] = c.box.hi[2];
     P.ref[side] = encode(u, nd.child + side);
   }
-  P.meta = (int32_t)nd.axis | ((nd.bits & 1) << 2);
+  P.meta = (int32_t)(nd.axis & 3) | ((nd.bits & 1) << 2);
   P.node = (int32_t)r;
 }
 

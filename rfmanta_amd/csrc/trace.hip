@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_packet(DevScene sc, DevRays ra
         const bool act = lane >= sb && lane < se;
         if (act) {
           float t, u, v;
-          if (prim_test<PRIM>(sc.tris, slot, r, minT, t, u, v)) {
+          if (prim_test<PRIM>(sc, slot, r, minT, t, u, v)) {
             minT = t;
             st = 1;
             pos = slot;

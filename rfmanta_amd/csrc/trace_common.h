@@ -10,9 +10,37 @@
 
 #include "mh_device.h"
 
+// Structured-buffer load (buffer_load_dwordx4 ... idxen): the record index goes to the
+// address unit as it is, so fetching part of record i of a table costs no vector address
+// arithmetic. Indices at or past the descriptor's record count read zeros.
+typedef int MhBufRsrc __attribute__((ext_vector_type(4)));
+typedef float MhBufF4 __attribute__((ext_vector_type(4)));
+__device__ MhBufF4 mh_struct_buffer_load_v4f32(MhBufRsrc rsrc, int index, int offset, int soffset,
+                                               int aux) __asm("llvm.amdgcn.struct.buffer.load.v4f32");
+
 namespace mh {
 
 constexpr float kEps = 1.e-4f;   // T_EPSILON
+
+// Descriptor of a table of `count` records of `stride` bytes (stride < 16384) at `base`:
+// 48-bit base, stride in word 1 bits 16-29, record count, gfx9 word 3 (32-bit data
+// format, no swizzle, no thread-id indexing).
+__device__ __forceinline__ MhBufRsrc record_table(const void* base, unsigned stride,
+                                                  unsigned count) {
+  const unsigned long long a = (unsigned long long)base;
+  MhBufRsrc r;
+  r.x = (int)(unsigned)a;
+  r.y = (int)(((unsigned)(a >> 32) & 0xFFFFu) | (stride << 16));
+  r.z = (int)count;
+  r.w = 0x00020000;
+  return r;
+}
+
+// 16 bytes at byte `offset` (a constant) of record `index`.
+__device__ __forceinline__ float4 fetch_record(const MhBufRsrc& t, int index, int offset) {
+  const MhBufF4 v = mh_struct_buffer_load_v4f32(t, index, offset, 0, 0);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 constexpr float kMaxT = 1.e19f;  // MAXT
 
 // Explicit address spaces. Pointers read from memory (batch tables) are otherwise
@@ -53,7 +81,9 @@ static __device__ __forceinline__ float pick3(int k, float a, float b, float c) 
 
 struct Ray {
   float o[3], d[3], inv[3];
-  unsigned dpos;  // bit k set iff d[k] > 0 (front child choice on split axis k)
+  // bit k and bit k + 4 set iff d[k] > 0 (front child choice on split axis k); the copy
+  // lets a PairNode's meta (axis | isLeftCheaper << 2) index the bit directly.
+  unsigned dpos;
 };
 
 // Slab test of DynBVH::firstIntersects (scalar loop, DynBVH.cc:666-680).
@@ -92,9 +122,11 @@ static __device__ __forceinline__ bool slab_fast(float lx, float hx, float ly, f
   const float ay = (ly - r.o[1]) * r.inv[1], by = (hy - r.o[1]) * r.inv[1];
   const float az = (lz - r.o[2]) * r.inv[2], bz = (hz - r.o[2]) * r.inv[2];
   const float tmin = fmaxf(fmaxf(fmaxf(kEps, fminf(ax, bx)), fminf(ay, by)), fminf(az, bz));
-  const float tmax = fminf(fminf(fminf(minT, fmaxf(ax, bx)), fmaxf(ay, by)), fmaxf(az, bz));
+  const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
   tnear = tmin;
-  return tmin <= tmax;
+  // tmin <= min(minT, tfar), written as two comparisons (no NaN here), so minT, which
+  // the loop carries, needs no canonicalising min.
+  return (tmin <= tfar) & (tmin <= minT);
 }
 
 template <bool FAST>
@@ -167,10 +199,11 @@ static __device__ __forceinline__ bool wald_test(const float4 A, const float4 B,
 }
 
 template <int PRIM>
-static __device__ __forceinline__ bool prim_test(const float* tris, int slot, const Ray& r, float minT,
-                                          float& t, float& u, float& v) {
-  const float4* p = reinterpret_cast<const float4*>(tris + 12 * (size_t)slot);
-  float4 A = p[0], B = p[1], C = p[2];
+static __device__ __forceinline__ bool prim_test(const DevScene& sc, int slot, const Ray& r,
+                                                 float minT, float& t, float& u, float& v) {
+  const MhBufRsrc tt = record_table(sc.tris, 12 * sizeof(float), sc.n_slots);
+  float4 A = fetch_record(tt, slot, 0), B = fetch_record(tt, slot, 16),
+         C = fetch_record(tt, slot, 32);
   // All 48 bytes are fetched before the first rejection test (one round trip) instead of
   // the compiler deferring the edge loads past the distance test.
   // (Input operands only: the values stay in the registers they were loaded into.)
@@ -193,6 +226,7 @@ static __device__ __forceinline__ void load_ray(const GRays& rays, int64_t i, Ra
   r.inv[2] = 1.0f / r.d[2];
   r.dpos = (unsigned)(r.d[0] > 0.f) | ((unsigned)(r.d[1] > 0.f) << 1) |
            ((unsigned)(r.d[2] > 0.f) << 2);
+  r.dpos |= r.dpos << 4;
 }
 
 static __device__ __forceinline__ void store_hit(const GHits& h, int64_t i, int32_t prim, float t,

@@ -76,7 +76,7 @@ __device__ __forceinline__ bool leaf(const DevScene& sc, int first, int cnt, con
   for (int k = 0; k < cnt; ++k) {
     if (STATS) ++ntri;
     float t, u, v;
-    if (prim_test<PRIM>(sc.tris, first + k, r, h.minT, t, u, v)) {
+    if (prim_test<PRIM>(sc, first + k, r, h.minT, t, u, v)) {
       h.minT = t;
       h.best = first + k;
       h.u = u;
@@ -99,15 +99,17 @@ __device__ __forceinline__ void trace_pair(const DevScene& sc, const RefNode& ro
   st.reset();
   while (go) {
     if (cur >= 0) {
-      const float4* np = reinterpret_cast<const float4*>(sc.pairs + cur);
-      const float4 A = np[0], B = np[1], C = np[2];
-      const int4 M = reinterpret_cast<const int4*>(np)[3];
+      const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
+      const float4 A = fetch_record(pt, cur, 0), B = fetch_record(pt, cur, 16),
+                   C = fetch_record(pt, cur, 32), Mf = fetch_record(pt, cur, 48);
+      const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
+                               __float_as_int(Mf.w));
       float tn0, tn1;
       const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, r, h.minT, tn0);
       const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, r, h.minT, tn1);
       if (STATS) nbox += 2;
       // Front child: isLeftCheaper (any-hit) or the direction's sign on the split axis.
-      const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : ((r.dpos >> (M.z & 3)) & 1) == 0;
+      const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : __builtin_amdgcn_ubfe(r.dpos, M.z, 1) == 0;
       // Both hit: enter the front child, push the other. One hits: enter it. Written on
       // the hit masks, so the choice stays in scalar mask operations.
       if (h0 && h1) st.push(one_first ? M.x : M.y, one_first ? tn0 : tn1);
