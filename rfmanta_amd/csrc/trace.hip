@@ -31,13 +31,13 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kPacketStack = 128;
 
 // ---------------------------------------------------------------------------
-// Single-ray traversal (traverse.h), grid-stride over the rays
+// Single-ray traversal (traverse.h), grid-stride over the rays. Eight waves per SIMD (the
+// scalar registers otherwise limit it to seven): more node fetches in flight per SIMD.
 // ---------------------------------------------------------------------------
 template <bool ANY, int PRIM, int LAYOUT, int S, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_trace_ray(DevScene sc, DevRays rays, DevHits hits,
-                                                      int64_t n, int2* spill,
-                                                      int64_t spill_stride,
-                                                      unsigned long long* counters) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_trace_ray(
+    DevScene sc, DevRays rays, DevHits hits, int64_t n, int2* spill, int64_t spill_stride,
+    unsigned long long* counters) {
   __shared__ unsigned long long s_ring[S * kBlock];
   const int tid = threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
