@@ -277,33 +277,46 @@ __device__ __forceinline__ int64_t used(const unsigned* next_free, int32_t n_rec
   return n < (unsigned)n_records ? (int64_t)n : (int64_t)n_records;
 }
 
-__device__ __forceinline__ int32_t encode(const DevUpdate& u, int rec) {
-  const RefNode& n = u.refs[rec];
-  if (n.children == 0) return (n.child - 1) >> 1;
-  const int c = n.children < kLeafCountInline ? n.children : kLeafCountInline;
-  if (n.children >= kLeafCountInline) u.big_count[n.child] = n.children;
-  return ~((n.child << kLeafCountBits) | c);
+// A record as two 16-byte words: lo.x lo.y lo.z hi.x | hi.y hi.z child word, where word
+// holds axis (byte 0), bits (byte 1) and children (int16, bytes 2-3).
+struct RecWords {
+  float4 a, b;
+};
+__device__ __forceinline__ RecWords load_rec(const RefNode* refs, int64_t r) {
+  const float4* p = reinterpret_cast<const float4*>(refs + r);
+  return RecWords{p[0], p[1]};
+}
+__device__ __forceinline__ int rec_child(const RecWords& w) { return __float_as_int(w.b.z); }
+__device__ __forceinline__ int rec_children(const RecWords& w) {
+  return __float_as_int(w.b.w) >> 16;  // arithmetic shift: the int16 with its sign
 }
 
+// Child reference of a PairNode for a loaded record: its pair index (inner) or its leaf
+// code, with big_count set for leaves of 15 or more objects.
+__device__ __forceinline__ int32_t encode_rec(const DevUpdate& u, const RecWords& w) {
+  const int children = rec_children(w), child = rec_child(w);
+  if (children == 0) return (child - 1) >> 1;
+  const int c = children < kLeafCountInline ? children : kLeafCountInline;
+  if (children >= kLeafCountInline) u.big_count[child] = children;
+  return ~((child << kLeafCountBits) | c);
+}
+
+// The PairNode of every inner record in use, written as four 16-byte stores from the two
+// child records read as four 16-byte loads.
 __global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (r >= used(u.next_free, u.n_records)) return;
-  const RefNode nd = u.refs[r];
-  if (nd.children != 0) return;
-  PairNode& P = u.pairs[(nd.child - 1) >> 1];
-  for (int side = 0; side < 2; ++side) {
-    const RefNode& c = u.refs[nd.child + side];
-    float* b = P.b + 6 * side;
-    b[0] = c.box.lo[0];
-    b[1] = c.box.hi[0];
-    b[2] = c.box.lo[1];
-    b[3] = c.box.hi[1];
-    b[4] = c.box.lo[2];
-    b[5] = c.box.hi[2];
-    P.ref[side] = encode(u, nd.child + side);
-  }
-  P.meta = (int32_t)(nd.axis & 3) | ((nd.bits & 1) << 2);
-  P.node = (int32_t)r;
+  const RecWords nd = load_rec(u.refs, r);
+  if (rec_children(nd) != 0) return;
+  const int c = rec_child(nd);
+  const RecWords c0 = load_rec(u.refs, c), c1 = load_rec(u.refs, c + 1);
+  float4* P = reinterpret_cast<float4*>(u.pairs + ((c - 1) >> 1));
+  P[0] = make_float4(c0.a.x, c0.a.w, c0.a.y, c0.b.x);  // child 0: lo.x hi.x lo.y hi.y
+  P[1] = make_float4(c0.a.z, c0.b.y, c1.a.x, c1.a.w);  // lo.z hi.z | child 1: lo.x hi.x
+  P[2] = make_float4(c1.a.y, c1.b.x, c1.a.z, c1.b.y);  // lo.y hi.y lo.z hi.z
+  const int word = __float_as_int(nd.b.w);
+  reinterpret_cast<int4*>(P)[3] =
+      make_int4(encode_rec(u, c0), encode_rec(u, c1), (word & 3) | (((word >> 8) & 1) << 2), (int)r);
 }
 
 // anc = parent record (root: -1), dep = 1 (root: 0) for every record in use.
@@ -346,18 +359,56 @@ __global__ __launch_bounds__(kBlock) void k_jump(const unsigned* next_free, int3
 // hist[d] = inner records at depth d (d < kMaxDepth), counted in LDS first (one global
 // atomic per bin and workgroup). hist[kMaxDepth] is set when an inner record lies at
 // kMaxDepth or deeper. Holes are skipped.
+// Adds one to bins[b] for every lane of the wave with b >= 0, with one LDS atomic per
+// distinct value; returns the lane's place among the lanes counted into its bin by this
+// call and earlier ones (lane order inside a call). Called by every lane of the wave.
+__device__ __forceinline__ unsigned wave_bin_add(unsigned* bins, int b) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(b >= 0);
+  unsigned place = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int v = __shfl(b, leader);
+    const unsigned long long m = __ballot(b == v);
+    unsigned before = 0;
+    if (lane == leader) before = atomicAdd(&bins[v], (unsigned)__popcll(m));
+    before = (unsigned)__shfl((int)before, leader);
+    if (b == v) place = before + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    todo &= ~m;
+  }
+  return place;
+}
+
+// Records [blockIdx.x * chunk, + chunk) of a workgroup (chunk a multiple of kBlock), cut
+// at the records in use.
+__device__ __forceinline__ void depth_range(const DevUpdate& u, int64_t chunk, int64_t& lo,
+                                            int64_t& hi) {
+  const int64_t n = used(u.next_free, u.n_records);
+  lo = (int64_t)blockIdx.x * chunk;
+  hi = lo + chunk < n ? lo + chunk : n;
+}
+
+// Inner records per depth. Each workgroup counts a range of records in LDS and adds its
+// counts to the global histogram once per depth, so the few shared counters see one
+// atomic per workgroup and depth.
 __global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* dep,
-                                                       unsigned* hist) {
+                                                       unsigned* hist, int64_t chunk) {
   __shared__ unsigned lh[kMaxDepth];
   for (int i = threadIdx.x; i < kMaxDepth; i += kBlock) lh[i] = 0;
   __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r < used(u.next_free, u.n_records) && u.refs[r].children == 0) {
-    const int d = dep[r];
-    if (d < kMaxDepth)
-      atomicAdd(&lh[d], 1u);
-    else
-      hist[kMaxDepth] = 1u;
+  int64_t lo, hi;
+  depth_range(u, chunk, lo, hi);
+  for (int64_t r0 = lo; r0 < hi; r0 += kBlock) {
+    const int64_t r = r0 + threadIdx.x;
+    int b = -1;
+    if (r < hi && u.refs[r].children == 0) {
+      const int d = dep[r];
+      if (d < kMaxDepth)
+        b = d;
+      else
+        hist[kMaxDepth] = 1u;
+    }
+    wave_bin_add(lh, b);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kMaxDepth; i += kBlock)
@@ -390,47 +441,56 @@ __global__ __launch_bounds__(kBlock) void k_depth_finish(unsigned* hist) {
 }
 
 // cursor[d]: next position for depth d in levels; cursor[kMaxDepth]: next leaf position.
-// Each workgroup reserves one range per depth and one for its leaves, then its records
-// take places inside them (order within a list does not matter). Inner depths are
-// < kMaxDepth (checked by the host before this pass).
+// Each workgroup counts its range of records per depth (leaves in bin kMaxDepth), reserves
+// one range per bin, then passes over its records again to write them into their ranges
+// (order within a list does not matter). Inner depths are < kMaxDepth (checked by the host
+// before this pass).
 __global__ __launch_bounds__(kBlock) void k_depth_scatter(DevUpdate u, const int32_t* dep,
                                                           unsigned* cursor, int32_t* levels,
-                                                          int32_t* leaves) {
-  __shared__ unsigned cnt[kMaxDepth], base[kMaxDepth], scan[kBlock];
-  __shared__ unsigned leaf_base;
-  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock) cnt[i] = 0;
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  int bin = -1;  // depth of an inner record, kMaxDepth for a leaf
-  if (r < used(u.next_free, u.n_records)) {
+                                                          int32_t* leaves, int64_t chunk) {
+  __shared__ unsigned cnt[kMaxDepth + 1], base[kMaxDepth + 1];
+  for (int i = threadIdx.x; i <= kMaxDepth; i += kBlock) cnt[i] = 0;
+  __syncthreads();
+  int64_t lo, hi;
+  depth_range(u, chunk, lo, hi);
+  auto bin_of = [&](int64_t r) {
+    if (r >= hi) return -1;
     const int16_t c = u.refs[r].children;
-    if (c > 0)
-      bin = kMaxDepth;
-    else if (c == 0 && dep[r] < kMaxDepth)
-      bin = dep[r];
+    if (c > 0) return kMaxDepth;
+    if (c == 0 && dep[r] < kMaxDepth) return (int)dep[r];
+    return -1;  // a hole
+  };
+  for (int64_t r0 = lo; r0 < hi; r0 += kBlock) wave_bin_add(cnt, bin_of(r0 + threadIdx.x));
+  __syncthreads();
+  for (int i = threadIdx.x; i <= kMaxDepth; i += kBlock) {
+    base[i] = cnt[i] ? atomicAdd(cursor + i, cnt[i]) : 0u;
+    cnt[i] = 0;
   }
-  const unsigned is_leaf = bin == kMaxDepth ? 1u : 0u;
-  scan[threadIdx.x] = is_leaf;
   __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive scan of the leaf flags
-    const unsigned x = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0u;
-    __syncthreads();
-    scan[threadIdx.x] += x;
-    __syncthreads();
+  for (int64_t r0 = lo; r0 < hi; r0 += kBlock) {
+    const int64_t r = r0 + threadIdx.x;
+    const int b = bin_of(r);
+    const unsigned place = wave_bin_add(cnt, b);
+    if (b == kMaxDepth)
+      leaves[base[b] + place] = (int32_t)r;
+    else if (b >= 0)
+      levels[base[b] + place] = (int32_t)r;
   }
-  unsigned local = 0;
-  if (bin >= 0 && bin < kMaxDepth) local = atomicAdd(&cnt[bin], 1u);
-  if (threadIdx.x == kBlock - 1) leaf_base = scan[kBlock - 1] ? atomicAdd(cursor + kMaxDepth, scan[kBlock - 1]) : 0u;
-  __syncthreads();
-  for (int i = threadIdx.x; i < kMaxDepth; i += kBlock)
-    if (cnt[i]) base[i] = atomicAdd(cursor + i, cnt[i]);
-  __syncthreads();
-  if (is_leaf)
-    leaves[leaf_base + scan[threadIdx.x] - 1] = (int32_t)r;
-  else if (bin >= 0)
-    levels[base[bin] + local] = (int32_t)r;
 }
 
 unsigned grid(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Workgroups of the depth histogram and scatter: at most kDepthBlocks, each over a range
+// of a multiple of kBlock records covering the n record slots.
+constexpr unsigned kDepthBlocks = 2048;
+int64_t depth_chunk(int64_t n, unsigned& blocks) {
+  const unsigned g = grid(n > 0 ? n : 1);
+  const unsigned want = g < kDepthBlocks ? g : kDepthBlocks;
+  const int64_t chunk = ((int64_t)g + want - 1) / want * kBlock;
+  blocks = (unsigned)((n + chunk - 1) / chunk);
+  if (blocks == 0) blocks = 1;
+  return chunk;
+}
 
 }  // namespace
 
@@ -475,15 +535,19 @@ int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc
                        even ? anc0 : anc1,
                        even ? dep0 : dep1, even ? anc1 : anc0, even ? dep1 : dep0);
   }
-  hipLaunchKernelGGL(k_depth_hist, dim3(g), dim3(kBlock), 0, s, u, dep0, hist);
+  unsigned blocks;
+  const int64_t chunk = depth_chunk(u.n_records, blocks);
+  hipLaunchKernelGGL(k_depth_hist, dim3(blocks), dim3(kBlock), 0, s, u, dep0, hist, chunk);
   hipLaunchKernelGGL(k_depth_finish, dim3(1), dim3(kBlock), 0, s, hist);
   return (int)hipGetLastError();
 }
 
 int launch_depth_scatter(const DevUpdate& u, const int32_t* dep, unsigned* cursor,
                          int32_t* levels, int32_t* leaves, void* stream) {
-  hipLaunchKernelGGL(k_depth_scatter, dim3(grid(u.n_records)), dim3(kBlock), 0,
-                     reinterpret_cast<hipStream_t>(stream), u, dep, cursor, levels, leaves);
+  unsigned blocks;
+  const int64_t chunk = depth_chunk(u.n_records, blocks);
+  hipLaunchKernelGGL(k_depth_scatter, dim3(blocks), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), u, dep, cursor, levels, leaves, chunk);
   return (int)hipGetLastError();
 }
 
