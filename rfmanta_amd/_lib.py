@@ -14,6 +14,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libmh.so"
 
 MH_OK = 0
+MH_E_INVALID, MH_E_HIP, MH_E_NOMEM, MH_E_STATE, MH_E_UNSUPPORTED, MH_E_IO = -1, -2, -3, -4, -5, -6
 MH_CLOSEST_HIT = 0
 MH_ANY_HIT = 1
 MH_PRIM_KENSLER_SHIRLEY = 0

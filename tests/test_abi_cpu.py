@@ -110,3 +110,19 @@ def test_file_format_round_trip(tmp_path, oracle):
     n_ids = int(np.frombuffer(raw[:4], np.uint32)[0])
     back = np.frombuffer(raw[8 + 4 * n_ids:], L.NODE_DTYPE)
     assert back.tobytes() == nodes.tobytes()
+
+
+def test_triangle_limit_is_rejected_before_any_work(mh):
+    """2^26 triangles and more are refused (leaf codes and the pair table's descriptor reach)
+    before the index array is read or a device is touched."""
+    v = np.zeros(3, np.float32)
+    i = np.zeros(3, np.uint32)
+    out = C.c_void_p()
+    assert mh.mh_scene_create(v.ctypes.data, 1, i.ctypes.data, 1 << 26, C.byref(out)) == L.MH_E_UNSUPPORTED
+    assert out.value is None
+    ids = np.zeros(1, np.int32)
+    nodes = np.zeros(64, np.uint8)
+    n = C.c_uint32()
+    assert mh.mh_dynbvh_build_host(v.ctypes.data, 1, i.ctypes.data, 1 << 26, None, ids.ctypes.data,
+                                   nodes.ctypes.data, C.byref(n)) == L.MH_E_UNSUPPORTED
+    assert b"exceeds the limit" in mh.mh_last_error()
