@@ -574,7 +574,19 @@ def cpu_baseline(verts, idx, scene, prim_rays, bounce) -> dict:
         total_rays += n * done
         total_s += spent
         log(f"cpu baseline {name}: {done} x {n} rays in {spent:.2f}s on {cores} threads")
-    return {"value": round(total_rays / total_s / 1e6, 3), "unit": "Mrays/s", "cores": cores,
+    # Thread scaling: one pass over the same rays on one thread (about a second).
+    one_rays, one_s = 0, 0.0
+    for rays in (prim_rays, bounce):
+        sub = {k: rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
+        t0 = time.perf_counter()
+        pyoracle.trace(verts, idx, nodes, ids, sub, packet_size=64, threads=1)
+        one_s += time.perf_counter() - t0
+        one_rays += len(sub["ox"])
+    value = total_rays / total_s / 1e6
+    one = one_rays / one_s / 1e6
+    log(f"cpu baseline one thread: {one:.2f} Mrays/s ({value / one:.1f}x on {cores} threads)")
+    return {"value": round(value, 3), "unit": "Mrays/s", "cores": cores,
+            "one_thread": round(one, 3), "thread_scaling": round(value / one, 2),
             "kind": "port",
             "sample": f"the full frame's {len(prim_rays['ox'])} primary rays x{reps['primary']} "
                       f"and {len(bounce['ox'])} bounce rays x{reps['bounce']}, 64-ray packets "
