@@ -574,14 +574,15 @@ def cpu_baseline(verts, idx, scene, prim_rays, bounce) -> dict:
         total_rays += n * done
         total_s += spent
         log(f"cpu baseline {name}: {done} x {n} rays in {spent:.2f}s on {cores} threads")
-    # Thread scaling: one pass over the same rays on one thread (about a second).
+    # Thread scaling: one pass over each ray set on one thread (about a second), weighted
+    # like the sample above (the same number of passes of each set).
     one_rays, one_s = 0, 0.0
-    for rays in (prim_rays, bounce):
+    for name, rays in (("primary", prim_rays), ("bounce", bounce)):
         sub = {k: rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
         t0 = time.perf_counter()
         pyoracle.trace(verts, idx, nodes, ids, sub, packet_size=64, threads=1)
-        one_s += time.perf_counter() - t0
-        one_rays += len(sub["ox"])
+        one_s += (time.perf_counter() - t0) * reps[name]
+        one_rays += len(sub["ox"]) * reps[name]
     value = total_rays / total_s / 1e6
     one = one_rays / one_s / 1e6
     log(f"cpu baseline one thread: {one:.2f} Mrays/s ({value / one:.1f}x on {cores} threads)")
