@@ -40,6 +40,55 @@ def canonical_equal(ga, gi, oa, oi):
     return True, "", n
 
 
+def canonical_equal_levels(ga, gi, oa, oi):
+    """canonical_equal, a level at a time with numpy (for trees of a million records)."""
+    g = np.zeros(1, np.int64)
+    o = np.zeros(1, np.int64)
+    n = 0
+    while len(g):
+        n += len(g)
+        for f in ("lo", "hi", "axis", "bits", "children"):
+            if ga[f][g].tobytes() != oa[f][o].tobytes():
+                return False, f"a level differs in {f}", n
+        c = ga["children"][g].astype(np.int64)
+        leaf = c > 0
+        for gg, oo, cc in zip(ga["child"][g[leaf]], oa["child"][o[leaf]], c[leaf]):
+            if not np.array_equal(gi[gg:gg + cc], oi[oo:oo + cc]):
+                return False, f"leaf ids differ at slot {gg}", n
+        gc = ga["child"][g[~leaf]].astype(np.int64)
+        oc = oa["child"][o[~leaf]].astype(np.int64)
+        g = np.concatenate([gc, gc + 1])
+        o = np.concatenate([oc, oc + 1])
+    return True, "", n
+
+
+def test_update_large_tree_matches_oracle(oracle):
+    """A 418k-triangle heightfield: the depth histogram and scatter run several record
+    ranges per workgroup at this size."""
+    print("update large heightfield", flush=True)
+    v, i = scenes.heightfield(700, 300, seed=5)
+    nodes, ids, nn = oracle.build_bvh(v, i)
+    st = oracle.RotState(nodes, ids, nn)
+    rays = scenes.random_rays(8192, 9, (-1.3, -1.3, -1.3), (1.3, 1.3, 1.3))
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes)
+        for f in range(2):
+            vd = v.copy()
+            vd[:, 1] += (0.03 * np.sin(4 * v[:, 0] + 0.9 * f)).astype(np.float32)
+            st.update(vd, i)
+            s.set_vertices(vd)
+            s.update()
+            api.sync()
+            gi, gn = s.export_dynbvh()
+            ok, msg, count = canonical_equal_levels(gn, gi, st.nodes, st.ids)
+            assert ok, (f, msg)
+            assert count == st.next_free.value
+            g = s.trace_host(rays)
+            o = oracle.trace(vd, i, st.nodes, st.ids, rays, packet_size=1)
+            assert np.array_equal(g["prim"], o["prim"]), f
+            assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32)), f
+
+
 @pytest.mark.parametrize("mesh", ["heightfield", "soup"])
 def test_update_matches_oracle(oracle, mesh):
     if mesh == "heightfield":
