@@ -83,6 +83,7 @@ struct Lane {
   int bi;      // batch index
   int64_t li;  // ray index inside the batch
   bool any;
+  bool regular;  // slab_fast is exact for this ray (regular_ray at its start)
 };
 
 // Per-lane refill schedule, REF layout: one traversal step of traverse.h's trace_ref with
@@ -111,69 +112,52 @@ __device__ __forceinline__ bool step_ref(const DevScene& sc, Lane& L, LaneStack<
   return true;
 }
 
-// Per-lane refill schedule, PAIR layout: the walk of traverse.h's trace_pair, resumed
-// from the lane's state and left after each leaf so that the wave can refill lanes whose
-// rays have finished. Returns false when the ray is finished.
+// Per-lane refill schedule, PAIR layout: one step of traverse.h's trace_pair loop (an inner
+// node, or a leaf and the pops after it), resumed from the lane's state, so that the wave
+// can refill lanes whose rays have finished between any two steps. Returns false when the
+// ray is finished.
 template <bool ANY, int PRIM, int S, int BLOCK, bool FAST>
-__device__ __forceinline__ bool walk_pair(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
+__device__ __forceinline__ bool step_pair(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
   unsigned long long ntri = 0;
-  for (;;) {
-    bool at_leaf = false;
-    if (L.cur >= 0) {
-      const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
-      const float4 A = fetch_record(pt, L.cur, 0), B = fetch_record(pt, L.cur, 16),
-                   C = fetch_record(pt, L.cur, 32), Mf = fetch_record(pt, L.cur, 48);
-      const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
-                               __float_as_int(Mf.w));
-      float tn0, tn1;
-      const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
-      const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
-      const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : __builtin_amdgcn_ubfe(L.r.dpos, M.z, 1) == 0;
-      const int fref = one_first ? M.y : M.x;
-      const int bref = one_first ? M.x : M.y;
-      const bool fh = one_first ? h1 : h0;
-      const bool bh = one_first ? h0 : h1;
-      if (fh) {
-        if (bh) st.push(bref, one_first ? tn0 : tn1);
-        L.cur = fref;
-        continue;
-      }
-      if (bh) {
-        L.cur = bref;
-        continue;
-      }
-    } else {
-      const int code = ~L.cur;
-      const int first = code >> kLeafCountBits;
-      int cnt = code & kLeafCountInline;
-      if (cnt == kLeafCountInline) cnt = sc.big_count[first];
-      if (leaf<ANY, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)) return false;
-      at_leaf = true;
+  if (L.cur >= 0) {
+    const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
+    const float4 A = fetch_record(pt, L.cur, 0), B = fetch_record(pt, L.cur, 16),
+                 C = fetch_record(pt, L.cur, 32), Mf = fetch_record(pt, L.cur, 48);
+    const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
+                             __float_as_int(Mf.w));
+    float tn0, tn1;
+    const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
+    const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
+    const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : __builtin_amdgcn_ubfe(L.r.dpos, M.z, 1) == 0;
+    if (h0 && h1) st.push(one_first ? M.x : M.y, one_first ? tn0 : tn1);
+    if (h0 || h1) {
+      L.cur = (h1 && (one_first || !h0)) ? M.y : M.x;
+      return true;
     }
-    bool go = false;
-    while (!st.empty()) {
-      int ref;
-      float t;
-      st.pop(ref, t);
-      if (t <= L.h.minT) {
-        L.cur = ref;
-        go = true;
-        break;
-      }
-    }
-    if (!go) return false;
-    if (at_leaf) return true;
+  } else {
+    const int code = ~L.cur;
+    const int first = code >> kLeafCountBits;
+    int cnt = code & kLeafCountInline;
+    if (cnt == kLeafCountInline) cnt = sc.big_count[first];
+    if (leaf<ANY, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)) return false;
   }
+  while (!st.empty()) {
+    int ref;
+    float t;
+    st.pop(ref, t);
+    if (t <= L.h.minT) {
+      L.cur = ref;
+      return true;
+    }
+  }
+  return false;
 }
 
-// Starts ray g (global number); returns false if it finished immediately.
+// Starts ray li of batch B in lane L; returns false if it finished immediately.
 template <int PRIM, int LAYOUT, int S, int BLOCK>
-__device__ __forceinline__ bool start_ray(const DevScene& sc, const RefNode& root,
-                                          const BatchDev* batches, const int64_t* s_start, int nb,
-                                          int64_t g, Lane& L, LaneStack<S, BLOCK>& st) {
-  L.bi = find_batch(s_start, nb, g);
-  L.li = g - s_start[L.bi];
-  const BatchDev& B = batches[L.bi];
+__device__ __forceinline__ bool start_in(const DevScene& sc, const RefNode& root, const BatchDev& B,
+                                         int64_t li, Lane& L, LaneStack<S, BLOCK>& st) {
+  L.li = li;
   L.any = (B.flags & 1u) != 0;
   if (B.rays.masked && B.rays.masked[L.li]) {
     store_hit(B.hits, L.li, -2, -kMaxT, 0.f, 0.f, true);
@@ -181,6 +165,7 @@ __device__ __forceinline__ bool start_ray(const DevScene& sc, const RefNode& roo
   }
   load_ray(B.rays, L.li, L.r);
   L.h = RayHit{B.rays.tmax[L.li], -1, 0.f, 0.f};
+  L.regular = regular_ray(L.r, L.h.minT);  // minT only falls to finite hit distances
   st.reset();
   if (LAYOUT == kLayoutPair) {
     float tn;
@@ -196,11 +181,33 @@ __device__ __forceinline__ bool start_ray(const DevScene& sc, const RefNode& roo
   return true;
 }
 
-__device__ __forceinline__ void finish(const BatchDev* batches, const DevScene& sc, const Lane& L) {
-  const BatchDev& B = batches[L.bi];
+// Starts ray g (global number); returns false if it finished immediately. When every lane
+// starting a ray takes it from the same batch (the usual case), the batch's record is read
+// once for the wave by scalar loads instead of once per lane.
+template <int PRIM, int LAYOUT, int S, int BLOCK>
+__device__ __forceinline__ bool start_ray(const DevScene& sc, const RefNode& root,
+                                          const BatchDev* batches, const int64_t* s_start, int nb,
+                                          int64_t g, Lane& L, LaneStack<S, BLOCK>& st) {
+  L.bi = find_batch(s_start, nb, g);
+  const int b0 = __builtin_amdgcn_readfirstlane(L.bi);
+  if (__ballot(L.bi != b0) == 0ull)
+    return start_in<PRIM, LAYOUT, S, BLOCK>(sc, root, batches[b0], g - s_start[b0], L, st);
+  return start_in<PRIM, LAYOUT, S, BLOCK>(sc, root, batches[L.bi], g - s_start[L.bi], L, st);
+}
+
+__device__ __forceinline__ void finish_in(const BatchDev& B, const DevScene& sc, const Lane& L) {
   const bool hit = L.h.best >= 0;
   store_hit(B.hits, L.li, hit ? sc.obj_ids[L.h.best] : -1, (L.any && hit) ? kEps : L.h.minT,
             L.h.u, L.h.v, hit);
+}
+
+// Stores a finished ray's hit (batch record by scalar loads when the lanes agree on it).
+__device__ __forceinline__ void finish(const BatchDev* batches, const DevScene& sc, const Lane& L) {
+  const int b0 = __builtin_amdgcn_readfirstlane(L.bi);
+  if (__ballot(L.bi != b0) == 0ull)
+    finish_in(batches[b0], sc, L);
+  else
+    finish_in(batches[L.bi], sc, L);
 }
 
 // Seven waves per SIMD (registers fitted to it, without spills) instead of the six the
@@ -271,13 +278,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) v
     const int64_t C = chunk;
     Lane L;
     bool active = false;
+    // A finished ray's hit is stored when its lane takes the next ray (or at the end): on
+    // gfx9 one counter tracks loads and stores, so a store after every finish would make
+    // the next node fetch of the whole wave wait for it.
+    bool pending = false;
     bool exhausted = false;
     int64_t pool_next = 0, pool_end = 0;  // wave-uniform
 
     for (;;) {
       const unsigned long long idle = __ballot(!active);
       const int n_idle = __popcll(idle);
-      if (n_idle == 64 && exhausted) break;
+      if (n_idle == 64 && exhausted) {
+        if (pending) finish(batches, sc, L);
+        break;
+      }
       if (n_idle >= refill && !exhausted) {
         const int rank = __popcll(idle & ((1ull << lane) - 1ull));
         const int64_t avail = pool_end - pool_next;
@@ -300,34 +314,38 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) v
         } else {
           pool_next += n_idle;
         }
-        if (g >= 0) active = start_ray<PRIM, LAYOUT, S, BLOCK>(sc, root, batches, s_start, nb, g, L, st);
+        if (g >= 0) {
+          if (pending) finish(batches, sc, L);
+          pending = false;
+          active = start_ray<PRIM, LAYOUT, S, BLOCK>(sc, root, batches, s_start, nb, g, L, st);
+        }
       }
       if constexpr (LAYOUT == kLayoutPair) {
         // min/max slab test when every ray of the wave allows it (wave-uniform choice)
-        const bool fast = __ballot(active && !regular_ray(L.r, L.h.minT)) == 0ull;
+        const bool fast = __ballot(active && !L.regular) == 0ull;
         if (active) {
           bool more;
           if (KIND == kKindClosest)
-            more = fast ? walk_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
-                        : walk_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? step_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
+                        : step_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
           else if (KIND == kKindAny)
-            more = fast ? walk_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
-                        : walk_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? step_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
+                        : step_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
           else if (L.any)
-            more = fast ? walk_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
-                        : walk_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? step_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
+                        : step_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
           else
-            more = fast ? walk_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
-                        : walk_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? step_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
+                        : step_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
           if (!more) {
-            finish(batches, sc, L);
             active = false;
+            pending = true;
           }
         }
       } else if (active) {
         if (!step_ref<PRIM, S, BLOCK>(sc, L, st)) {
-          finish(batches, sc, L);
           active = false;
+          pending = true;
         }
       }
     }
@@ -382,7 +400,7 @@ int launch_k(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
 // refill 64: whole-wave chunks (chunk 0: static stride); otherwise per-lane refill.
 template <int PRIM, int LAYOUT, int BLOCK>
 int launch_s(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
-  if (bl.refill < 64) return launch_w<PRIM, LAYOUT, BLOCK, kModeLane, kKindMixed>(sc, bl, s);
+  if (bl.refill < 64) return launch_k<PRIM, LAYOUT, BLOCK, kModeLane>(sc, bl, s);
   return bl.chunk == 0 ? launch_k<PRIM, LAYOUT, BLOCK, kModeWaveStatic>(sc, bl, s)
                        : launch_k<PRIM, LAYOUT, BLOCK, kModeWave>(sc, bl, s);
 }
