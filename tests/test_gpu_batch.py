@@ -82,7 +82,15 @@ def test_frame_matches_oracle(oracle, small, cfg):
         f.close()
 
 
-def test_graph_replays_and_table_rewrite(small):
+LANE_CFG = dict(layout=0, stack_lds=8, block=64, blocks_per_cu=24, refill=32, chunk=64,
+                xcd_remap=0)
+
+
+@pytest.mark.parametrize("cfg", [None, LANE_CFG], ids=["default", "lane_refill"])
+def test_graph_replays_and_table_rewrite(small, cfg):
+    """Replays of a captured frame (whole-wave and per-lane refill schedules, whose work
+    counters reset themselves) equal the eager results, with eager work in between."""
+    print(f"graph replays cfg={cfg}", flush=True)
     v, i, batches = small
     with api.Scene(v, i) as s:
         s.build()
@@ -96,6 +104,8 @@ def test_graph_replays_and_table_rewrite(small):
         stream = api.stream_create()
         try:
             f = api.Frame(s, dev)
+            if cfg is not None:
+                f.set_config(cfg)
             f.capture(stream)
             for rep in range(3):
                 for dr, dh, any_hit in dev:  # scribble over the outputs between replays
