@@ -110,8 +110,9 @@ bool read_file(const std::string& path, std::string& out) {
 // Corner forms, as glm tells them apart by the face's first corner.
 enum Form { kV, kVT, kVTN, kVN };
 
-// Parses one corner token in the given form; false when the token does not start with an
-// integer (the face ends before it), -1 when it starts like one but does not match.
+// Parses one corner token in the given form: 1 when it matches, 0 when the token does not
+// start with an integer (the face ends before it), -1 when it starts like one but does not
+// match the form.
 int corner(const std::string& tok, Form f, long& v, long& t, long& n) {
   const char* c = tok.c_str();
   char* end;
