@@ -419,7 +419,7 @@ int build_device(mh_scene* s, int rtsah) {
       (rc = s->obox.reserve((size_t)T * 24)) || (rc = s->ocen.reserve((size_t)T * 12)) ||
       (rc = s->anc0.reserve(nrec * 4)) || (rc = s->dep0.reserve(nrec * 4)) ||
       (rc = s->anc1.reserve(nrec * 4)) || (rc = s->dep1.reserve(nrec * 4)) ||
-      (rc = s->hist.reserve((mh::kMaxDepth + 2) * 4)) || (rc = s->counters.reserve(16)))
+      (rc = s->hist.reserve(mh::kDepthHist * 4)) || (rc = s->counters.reserve(16)))
     return rc;
   if (!s->logtab.p) {
     std::vector<float> tab(mh::rtsah::kLogTable);
@@ -511,11 +511,11 @@ int build_device(mh_scene* s, int rtsah) {
   HIP_OK((hipError_t)mh::build_launch_serial(d, back[mh::kCtrSmall], back[mh::kCtrBig],
                                              scratch.as<int32_t>(), st));
   // Depths of the temporary tree, then its per-depth lists.
-  HIP_OK(hipMemsetAsync(s->hist.p, 0, (mh::kMaxDepth + 2) * 4, st));
+  HIP_OK(hipMemsetAsync(s->hist.p, 0, mh::kDepthHist * 4, st));
   HIP_OK((hipError_t)mh::launch_depths(d.u, s->anc0.as<int32_t>(), s->dep0.as<int32_t>(),
                                        s->anc1.as<int32_t>(), s->dep1.as<int32_t>(),
-                                       s->hist.as<unsigned>(), st));
-  std::vector<unsigned> hist(mh::kMaxDepth + 2);
+                                       s->hist.as<unsigned>(), 8, st));
+  std::vector<unsigned> hist(mh::kDepthHist);
   unsigned used[2];
   HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
   HIP_OK(hipMemcpyAsync(used, cnt, 8, hipMemcpyDeviceToHost, st));
@@ -803,7 +803,7 @@ int mh_scene_update(mh_scene* s, void* stream) {
   if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->obox.reserve((size_t)T * 24)) ||
       (rc = s->ocen.reserve((size_t)T * 12)) || (rc = s->anc0.reserve(nrec * 4)) ||
       (rc = s->dep0.reserve(nrec * 4)) || (rc = s->anc1.reserve(nrec * 4)) ||
-      (rc = s->dep1.reserve(nrec * 4)) || (rc = s->hist.reserve((mh::kMaxDepth + 2) * 4)) ||
+      (rc = s->dep1.reserve(nrec * 4)) || (rc = s->hist.reserve(mh::kDepthHist * 4)) ||
       (rc = s->counters.reserve(16)))
     return rc;
   unsigned* nfree = s->counters.as<unsigned>();
@@ -839,16 +839,24 @@ int mh_scene_update(mh_scene* s, void* stream) {
   r.n_slots = (int32_t)T;
   r.prim_kind = s->prim_kind;
   HIP_OK((hipError_t)mh::launch_prim_setup(r, st));
-  // Depth lists of the new topology, for the next refit or update.
-  HIP_OK(hipMemsetAsync(s->hist.p, 0, (mh::kMaxDepth + 2) * 4, st));
-  HIP_OK((hipError_t)mh::launch_depths(u, s->anc0.as<int32_t>(), s->dep0.as<int32_t>(),
-                                       s->anc1.as<int32_t>(), s->dep1.as<int32_t>(),
-                                       s->hist.as<unsigned>(), st));
-  std::vector<unsigned> hist(mh::kMaxDepth + 2);
+  // Depth lists of the new topology, for the next refit or update. Pointer jumping takes
+  // enough steps for the previous depth with eight levels to spare (an even count); if the
+  // tree grew deeper than that, the pass says so and runs again with all eight steps.
+  int steps = 2;
+  while (steps < 8 && (1u << steps) <= s->depth + 8) steps += 2;
+  std::vector<unsigned> hist(mh::kDepthHist);
   unsigned back[2];
-  HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(back, nfree, 8, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipStreamSynchronize(st));
+  for (;;) {
+    HIP_OK(hipMemsetAsync(s->hist.p, 0, mh::kDepthHist * 4, st));
+    HIP_OK((hipError_t)mh::launch_depths(u, s->anc0.as<int32_t>(), s->dep0.as<int32_t>(),
+                                         s->anc1.as<int32_t>(), s->dep1.as<int32_t>(),
+                                         s->hist.as<unsigned>(), steps, st));
+    HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(back, nfree, 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (!hist[mh::kMaxDepth + 2] || steps == 8) break;
+    steps = 8;
+  }
   // Either failure leaves a device tree that must not be traced: the scene needs a build.
   if (back[1]) {
     s->built = s->updatable = false;

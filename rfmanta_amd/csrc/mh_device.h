@@ -165,10 +165,13 @@ int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leav
                        void* stream);
 // PairNodes and big leaf counts of every record pair in use, from the reference records.
 int launch_rebuild_pairs(const DevUpdate& u, void* stream);
-// Depth of every record in use; hist[d] inner records at depth d, hist[kMaxDepth] leaves,
-// hist[kMaxDepth + 1] the deepest depth (hist cleared by the caller). Depths in dep0.
+// Depth of every record in use by `steps` pointer-jumping steps (even, at most 8; 8 covers
+// any depth below kMaxDepth); hist[d] inner records at depth d, hist[kMaxDepth] leaves,
+// hist[kMaxDepth + 1] the deepest depth, hist[kMaxDepth + 2] nonzero when the steps did
+// not reach the root (hist, kDepthHist entries, cleared by the caller). Depths in dep0.
+constexpr int kDepthHist = kMaxDepth + 3;
 int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc1, int32_t* dep1,
-                  unsigned* hist, void* stream);
+                  unsigned* hist, int steps, void* stream);
 int launch_depth_scatter(const DevUpdate& u, const int32_t* dep, unsigned* cursor,
                          int32_t* levels, int32_t* leaves, void* stream);
 

@@ -391,8 +391,9 @@ __device__ __forceinline__ void depth_range(const DevUpdate& u, int64_t chunk, i
 // Inner records per depth. Each workgroup counts a range of records in LDS and adds its
 // counts to the global histogram once per depth, so the few shared counters see one
 // atomic per workgroup and depth.
-__global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* dep,
-                                                       unsigned* hist, int64_t chunk) {
+__global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_t* anc,
+                                                       const int32_t* dep, unsigned* hist,
+                                                       int64_t chunk) {
   __shared__ unsigned lh[kMaxDepth];
   for (int i = threadIdx.x; i < kMaxDepth; i += kBlock) lh[i] = 0;
   __syncthreads();
@@ -401,6 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_depth_hist(DevUpdate u, const int32_
   for (int64_t r0 = lo; r0 < hi; r0 += kBlock) {
     const int64_t r = r0 + threadIdx.x;
     int b = -1;
+    if (r < hi && anc[r] != -1) hist[kMaxDepth + 2] = 1u;  // the jumps did not reach the root
     if (r < hi && u.refs[r].children == 0) {
       const int d = dep[r];
       if (d < kMaxDepth)
@@ -524,12 +526,15 @@ int launch_rebuild_pairs(const DevUpdate& u, void* stream) {
 }
 
 int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc1, int32_t* dep1,
-                  unsigned* hist, void* stream) {
+                  unsigned* hist, int steps, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const unsigned g = grid(u.n_records);
   hipLaunchKernelGGL(k_parents, dim3(g), dim3(kBlock), 0, s, u, anc0, dep0);
-  // 2^8 > kMaxDepth: eight doublings reach the root from any record
-  for (int it = 0; it < 8; ++it) {
+  // 2^8 > kMaxDepth: eight doublings reach the root from any record; fewer (an even count,
+  // so that the results end in anc0/dep0) do for shallower trees, and the histogram pass
+  // flags a record they left short of the root.
+  if (steps < 2 || steps > 8 || (steps & 1)) steps = 8;
+  for (int it = 0; it < steps; ++it) {
     const bool even = (it & 1) == 0;
     hipLaunchKernelGGL(k_jump, dim3(g), dim3(kBlock), 0, s, u.next_free, u.n_records,
                        even ? anc0 : anc1,
@@ -537,7 +542,7 @@ int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc
   }
   unsigned blocks;
   const int64_t chunk = depth_chunk(u.n_records, blocks);
-  hipLaunchKernelGGL(k_depth_hist, dim3(blocks), dim3(kBlock), 0, s, u, dep0, hist, chunk);
+  hipLaunchKernelGGL(k_depth_hist, dim3(blocks), dim3(kBlock), 0, s, u, anc0, dep0, hist, chunk);
   hipLaunchKernelGGL(k_depth_finish, dim3(1), dim3(kBlock), 0, s, hist);
   return (int)hipGetLastError();
 }
