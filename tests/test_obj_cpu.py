@@ -123,3 +123,28 @@ def test_obj_errors(tmp_path):
         (tmp_path / name).write_text(text)
         with pytest.raises(L.MhError):
             api.read_obj(tmp_path / name)
+
+
+def test_obj_and_ply_of_one_mesh_agree(tmp_path):
+    """The same polygons written as OBJ (no groups) and as PLY read back to the same Mesh:
+    both readers fan polygons the same way and remove the same collapsed triangles."""
+    import struct
+    rng = np.random.default_rng(6)
+    v = rng.normal(size=(50, 3)).astype(np.float32)
+    v[7] = v[8]
+    faces = [list(rng.choice(50, size=int(k), replace=False)) for k in rng.integers(3, 7, 40)]
+    faces += [[7, 8, 9], [9, 8, 7, 10]]
+    obj = [f"v {a!r} {b!r} {c!r}" for a, b, c in v.tolist()]
+    obj += ["f " + " ".join(str(x + 1) for x in f) for f in faces]
+    (tmp_path / "m.obj").write_text("\n".join(obj) + "\n")
+    head = (f"ply\nformat binary_little_endian 1.0\nelement vertex {len(v)}\nproperty float x\n"
+            f"property float y\nproperty float z\nelement face {len(faces)}\n"
+            "property list uchar int vertex_indices\nend_header\n").encode()
+    body = b"".join(struct.pack("<3f", *p) for p in v.tolist())
+    body += b"".join(struct.pack("<B", len(f)) + struct.pack(f"<{len(f)}i", *f) for f in faces)
+    (tmp_path / "m.ply").write_bytes(head + body)
+    m = api.read_obj(tmp_path / "m.obj")
+    pv, pi = api.read_ply(tmp_path / "m.ply")
+    assert np.array_equal(m["verts"], pv)
+    assert np.array_equal(m["idx"], pi)
+    assert len(pi) < sum(len(f) - 2 for f in faces)  # collapsed triangles were removed
