@@ -212,9 +212,11 @@ __device__ __forceinline__ void finish(const BatchDev* batches, const DevScene& 
 
 // Seven waves per SIMD (registers fitted to it, without spills) instead of the six the
 // unconstrained allocation leaves: more node fetches in flight. Eight would spill to scratch
-// and measured slower.
+// and measured slower. The per-lane refill kernel for mixed tables keeps six: it carries
+// both ray kinds' steps and would spill at seven.
 template <int PRIM, int LAYOUT, int S, int BLOCK, int MODE, int KIND>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_batch(
+__global__ __launch_bounds__(BLOCK) __attribute__((
+    amdgpu_waves_per_eu(MODE == kModeLane && KIND == kKindMixed ? 6 : 7, 8))) void k_trace_batch(
     DevScene sc, const BatchTable* __restrict__ table, unsigned long long* ctr, int2* spill,
     int64_t spill_stride, int nb_launch, int refill, int chunk, int xcd_remap) {
   __shared__ unsigned long long s_ring[S * BLOCK];
