@@ -546,6 +546,9 @@ struct Cost2 {
   float c[2];
 };
 
+// Per record (c0, c1, lCost, rCost) of the last subtreeCost pass, when a caller asks.
+float* g_cost_out = nullptr;
+
 // DynBVH.cc:1594-1678 with opaque materials (canAttenuateShadows() == false).
 Cost2 subtreeCost(std::vector<Node32>& nodes, const TTP& ttp, int id) {
   const float ISEC = 3, TRAV = 1;
@@ -554,6 +557,12 @@ Cost2 subtreeCost(std::vector<Node32>& nodes, const TTP& ttp, int id) {
     Cost2 r;
     r.c[0] = node.children * ISEC + TRAV;
     r.c[1] = 0;
+    if (g_cost_out) {
+      float* o = g_cost_out + 4 * (size_t)id;
+      o[0] = r.c[0];
+      o[1] = r.c[1];
+      o[2] = o[3] = 0.f;
+    }
     return r;
   }
   float Plr, Pjl, Pjr;
@@ -575,6 +584,13 @@ Cost2 subtreeCost(std::vector<Node32>& nodes, const TTP& ttp, int id) {
     node.bits |= 1;
   else
     node.bits &= ~1;
+  if (g_cost_out) {
+    float* o = g_cost_out + 4 * (size_t)id;
+    o[0] = cost.c[0];
+    o[1] = cost.c[1];
+    o[2] = lCost;
+    o[3] = rCost;
+  }
   return cost;
 }
 
@@ -1545,6 +1561,18 @@ void oracle_rtsah(void* nodes, int n_nodes) {
   std::vector<Node32> nv((Node32*)nodes, (Node32*)nodes + n_nodes);
   TTP ttp;
   subtreeCost(nv, ttp, 0);
+  std::memcpy(nodes, nv.data(), nv.size() * sizeof(Node32));
+}
+
+// The same pass, also writing per record (c0, c1, lCost, rCost) to costs4 (4 floats per
+// record; leaves: their (c0, c1) and zeros; records the walk does not reach: untouched).
+// c0/c1 are computeSubTreeTraversalCost's cost[0]/cost[1] (DynBVH.cc:1594-1678).
+void oracle_rtsah_costs(void* nodes, int n_nodes, float* costs4) {
+  std::vector<Node32> nv((Node32*)nodes, (Node32*)nodes + n_nodes);
+  TTP ttp;
+  g_cost_out = costs4;
+  subtreeCost(nv, ttp, 0);
+  g_cost_out = nullptr;
   std::memcpy(nodes, nv.data(), nv.size() * sizeof(Node32));
 }
 

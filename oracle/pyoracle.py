@@ -45,6 +45,8 @@ def load() -> C.CDLL:
         lib.oracle_partition_objects.argtypes = [P, C.c_int, C.c_float, P]
         lib.oracle_rtsah.restype = None
         lib.oracle_rtsah.argtypes = [P, C.c_int]
+        lib.oracle_rtsah_costs.restype = None
+        lib.oracle_rtsah_costs.argtypes = [P, C.c_int, P]
         lib.oracle_trace.restype = C.c_int
         lib.oracle_trace.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_int] + [P] * 8 + [
             C.c_int64, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P]
@@ -131,6 +133,14 @@ def partition_objects(keys, position):
 
 def rtsah(nodes):
     load().oracle_rtsah(_p(nodes), len(nodes))
+
+
+def rtsah_costs(nodes):
+    """DynBVH::computeTraversalCost in place on nodes; returns float32[n, 4] with
+    (c0, c1, lCost, rCost) per record (leaves: c0, c1, 0, 0)."""
+    out = np.zeros((len(nodes), 4), np.float32)
+    load().oracle_rtsah_costs(_p(nodes), len(nodes), _p(out))
+    return out
 
 
 def _rays(rays):
