@@ -46,6 +46,22 @@ HIT_OUT_ANY = 4
 BOX_BYTES = 32        # one reference BVHNode per box test
 TRI_BYTES = 48        # 3 x u32 index + 3 x 12-byte vertex per triangle test
 C3_SLO_MS = 100.0     # p99 objective for the sustained-load figure
+# fp32 operations of one box test and one triangle test as the reference's scalar code
+# states them. Slab test (DynBVH.cc:666-680): per axis two subtractions, two products, a
+# min and a max for the interval, a max and a min into [tmin, tmax]: 8 x 3, plus the final
+# comparison. Kensler-Shirley (KenslerShirleyTriangle.cc:336-377, edges and normal
+# precomputed): det 5, reciprocal 1, p0 - o 3, t 6, its range 3, the cross product 9, u 6
+# and its sign 1, v 6 and its checks 4, the closing interval test 2.
+BOX_OPS = 25
+TRI_OPS = 45
+# VALU peaks of MI355X (256 CUs x 4 SIMDs of 16 lanes, 2.4 GHz peak engine clock): one
+# fp32 operation per lane per cycle (not packed, not fused), and one wave64 instruction
+# per SIMD every four cycles.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+PROFILE_JSON = "profiles/counters_trace_batch_heightfield1M.json"
+KERNEL_SOURCES = ("batch_kernel.h", "traverse.h", "trace_common.h", "mh_device.h",
+                  "batch_ks_pair.hip")
 
 
 def log(msg: str) -> None:
@@ -53,11 +69,12 @@ def log(msg: str) -> None:
 
 
 def host_cores() -> int:
+    """Host cores this process may run on (all of them: no cap)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, n)
 
 
 def mrays(n, ms):
@@ -184,11 +201,18 @@ def main() -> int:
     for _ in range(args.warmup):
         frame.launch()
     api.sync()
+    # Outputs overwritten before the timed region: the hits read back after it are the
+    # timed launches' own results (checked against the oracle by the CPU leg).
+    for dh in (h_prim, h_b):
+        dh.arrays["prim"].upload(np.full(dh.n, 777, np.int32))
+        dh.arrays["t"].upload(np.full(dh.n, -1.0, np.float32))
+    api.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         frame.launch()
     api.sync()
     ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    frame_hits = {"primary": h_prim.download(), "bounce": h_b.download()}
     k_frame = frame.time(args.steps)  # HIP events on the launch stream
     value = mrays(n_prim + n_b, ms_step)
     single = {}
@@ -206,14 +230,28 @@ def main() -> int:
     log(f"batched: step {ms_step:.3f} ms ({value:.0f} Mrays/s), kernel {k_frame:.3f} ms; "
         f"single-set frames {single}")
 
-    # Roofline of the dominant kernel: the persistent batch kernel over the C1 frame.
+    # Roofline of the dominant kernel, the persistent batch kernel over the C1 frame. Its
+    # binding resource is the VALUs (the HBM traffic it causes is a small fraction of the
+    # HBM peak: node and triangle records are served from the L2s and the Infinity
+    # Cache). achieved = the algorithm's fp32 operations per launch (per ray: box tests x
+    # BOX_OPS + triangle tests x TRI_OPS, the tests counted live above) / the launch time,
+    # against one operation per lane per cycle on every SIMD lane of the chip.
+    alg_ops = (n_prim * (BOX_OPS * nb_p + TRI_OPS * nt_p) +
+               n_b * (BOX_OPS * nb_b + TRI_OPS * nt_b))
+    achieved_tops = alg_ops / (k_frame * 1e-3) / 1e12
     alg_bytes = n_prim * bytes_p + n_b * bytes_b
-    achieved_gbs = alg_bytes / (k_frame * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+    roofline = {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": VALU_PEAK_TOPS,
+                "unit": "Tops/s", "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),
+                "traffic": None,
                 "kernel": "k_trace_batch (primary + bounce batches, closest-hit, KS)",
-                "bytes_per_ray": {"primary": round(bytes_p, 1), "bounce": round(bytes_b, 1)},
-                "rays_per_launch": n_prim + n_b, "launch_ms": round(k_frame, 4)}
+                "ops_per_test": {"box": BOX_OPS, "triangle": TRI_OPS},
+                "ops_per_launch": int(alg_ops), "rays_per_launch": n_prim + n_b,
+                "launch_ms": round(k_frame, 4),
+                # what an uncached traversal would read (32 B per box test, 48 B per
+                # triangle test, ray in and hit out), as a rate over the same launch
+                "uncached_demand_gbs": round(alg_bytes / (k_frame * 1e-3) / 1e9, 1),
+                "uncached_bytes_per_ray": {"primary": round(bytes_p, 1),
+                                           "bounce": round(bytes_b, 1)}}
     if args.copy_probe_mb > 0:
         # A copy kernel of known size in the same process: the reference for scaling
         # FETCH_SIZE / WRITE_SIZE counter readings to bytes.
@@ -225,10 +263,29 @@ def main() -> int:
         log(f"copy probe: 3 x {nbytes} bytes")
         a.free()
         b.free()
-    traffic = load_traffic()
-    if traffic is not None:
-        roofline["traffic"] = traffic["bytes_per_launch"]
-        roofline["traffic_source"] = traffic["source"]
+    prof = load_profile()
+    roofline["profile"] = PROFILE_JSON
+    if prof is None:
+        roofline["profile_status"] = "missing"
+    elif prof["sources_sha256"] != kernel_sources_sha():
+        roofline["profile_status"] = "stale: the kernel sources changed after it was taken"
+    elif prof["config"] != tuned:
+        roofline["profile_status"] = f"taken at another launch configuration {prof['config']}"
+    else:
+        # Counters of this kernel, these sources and this launch configuration: HBM bytes
+        # per launch (FETCH_SIZE / WRITE_SIZE, copy-calibrated), VALU instructions issued
+        # and the lanes they had active. Rates use this run's launch time.
+        sec = k_frame * 1e-3
+        roofline["profile_status"] = "current"
+        roofline["traffic"] = int(prof["bytes_per_launch"])
+        roofline["hbm"] = {"achieved_gbs": round(prof["bytes_per_launch"] / sec / 1e9, 1),
+                           "peak_gbs": HBM_PEAK_GBS,
+                           "frac": round(prof["bytes_per_launch"] / sec / 1e9 / HBM_PEAK_GBS, 4)}
+        roofline["valu_issue"] = {
+            "insts_per_launch": int(prof["valu_insts_per_launch"]),
+            "frac": round(prof["valu_insts_per_launch"] / sec / VALU_ISSUE_PEAK, 4),
+            "lanes_active": round(prof["lanes_active"], 4)}
+        roofline["l2_hit"] = prof.get("l2_hit")
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
@@ -271,8 +328,11 @@ def main() -> int:
     if "c4" not in skip:
         out["c4_dynamic"] = bench_c4(api, scenes, args)
 
-    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(verts, idx, scene, prim_rays,
-                                                                bounce)
+    if args.no_cpu:
+        out["cpu_baseline"], out["parity"] = None, None
+    else:
+        out["cpu_baseline"], out["parity"] = cpu_leg(verts, idx, scene, prim_rays, bounce,
+                                                     frame_hits)
     print(json.dumps(out), flush=True)
     scene.close()
     return 0
@@ -381,28 +441,6 @@ def bench_c2(api, L, scenes, scene, verts, idx, args, incoherent=False) -> dict:
     return res
 
 
-def c3_requests(scenes_mod, n, seed=2):
-    """Request stream: (scene, camera kwargs) from PCG32(seed)."""
-    r = scenes_mod.pcg32(seed, np.arange(n, dtype=np.uint64), 6)
-    u = [scenes_mod.u01(r[k]) for k in range(6)]
-    res = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
-    out = []
-    for i in range(n):
-        sc = int(u[0][i] < 0.5)
-        xr, yr = res[min(3, int(u[1][i] * 4))]
-        spp = 1 + min(3, int(u[2][i] * 4))
-        az = 2 * np.pi * u[3][i]
-        el = 0.35 + 0.5 * u[4][i]
-        if sc == 0:  # heightfield in [-1,1] x [-0.5,0.5]
-            rad, look = 1.6, (0.0, 0.0, 0.0)
-        else:        # soup in [-4,4]^3
-            rad, look = 11.0, (0.0, 0.0, 0.0)
-        eye = (rad * np.cos(el) * np.cos(az), rad * np.sin(el), rad * np.cos(el) * np.sin(az))
-        out.append((sc, dict(eye=eye, lookat=look, up=(0, 1, 0), vfov_deg=40.0, xres=xr,
-                             yres=yr, spp=spp, seed=i)))
-    return out
-
-
 def bench_c3(api, scenes, scene_hf, args) -> dict:
     t0 = time.time()
     v, i = scenes.soup(8_000_000, seed=0, scale=4.0)
@@ -411,10 +449,10 @@ def bench_c3(api, scenes, scene_hf, args) -> dict:
     soup.build()
     log(f"C3: soup 8M tris generated+built in {time.time() - t0:.1f}s")
     srv = api.Server([scene_hf, soup], hbm_budget=8 << 30, host_budget=2 << 30,
-                     chunk_rays=1 << 21, max_active=64, shadows=True, light=(2.0, 6.0, -2.0))
+                     chunk_rays=1 << 21, max_active=64, shadows=True, light=scenes.C3_LIGHT)
     per_point = 64
     qps_list = (25, 50, 100, 200, 400)
-    reqs = c3_requests(scenes, per_point * (len(qps_list) + 1))
+    reqs = scenes.c3_requests(per_point * (len(qps_list) + 1))
     # warm-up: a few requests back to back
     for sc, kw in reqs[:8]:
         srv.wait(srv.submit(sc, api.camera(**kw)), results=False)
@@ -493,7 +531,7 @@ def bench_c4(api, scenes, args) -> dict:
         f"{host_s:.2f}s, device approximate {approx_ms} ms), {st}")
     srv = api.Server([sc], hbm_budget=2 << 30, host_budget=1 << 30, chunk_rays=1 << 22,
                      shadows=True, bounces=True, light=(2.0, 2.0, -2.0))
-    cam = api.camera((0.0, 0.7, -0.9), (0.0, 0.0, 0.0), (0, 1, 0), 30.0, 1920, 1080, seed=4)
+    cam = api.camera(**scenes.C4_CAMERA)
     refit_ms, frame_ms, rays, hits = [], [], 0, 0
     # The bounds-only refit on its own, for the refit bandwidth figure.
     for f in range(3):
@@ -542,56 +580,124 @@ def bench_c4(api, scenes, args) -> dict:
     return res
 
 
-def load_traffic():
-    """HBM bytes per launch of the dominant kernel, from the committed counter summary."""
-    p = REPO / "profiles" / "traffic_trace_batch_heightfield1M.json"
+def kernel_sources_sha() -> str:
+    """SHA-256 over the sources of the batch traversal kernel, in a fixed order."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update((REPO / "rfmanta_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()
+
+
+def load_profile():
+    """Counter summary of the dominant kernel (rocprofv3 --pmc runs of this script)."""
+    p = REPO / PROFILE_JSON
     if not p.exists():
         return None
     with open(p) as f:
         return json.load(f)
 
 
-def cpu_baseline(verts, idx, scene, prim_rays, bounce) -> dict:
-    """The oracle's packet traversal (8x8 packets, one thread per core) on a sample."""
+def tie_classes(p_ref, t_ref, p_gpu, t_gpu):
+    """Prim-id differences split into H1 (|dt| <= 1e-6 t: exact ties such as shared
+    edges), H2 (|dt| < 1e-4: the Kensler-Shirley acceptance band, where the first-visited
+    triangle wins) and others."""
+    mis = p_ref != p_gpu
+    dt = np.abs(t_ref.astype(np.float64) - t_gpu.astype(np.float64))
+    scale = np.maximum(np.abs(t_ref.astype(np.float64)), 1e-30)
+    h1 = mis & (dt <= 1e-6 * scale)
+    h2 = mis & ~h1 & (dt < 1e-4)
+    return int(h1.sum()), int(h2.sum()), int((mis & ~h1 & ~h2).sum())
+
+
+def cpu_leg(verts, idx, scene, prim_rays, bounce, frame_hits):
+    """The CPU baseline and, as its checker, the parity of the timed frame.
+
+    Baseline: Manta's multithreaded SSE path as the oracle restates it (the reference's
+    x86 build: 64-ray packets in 8x8-tile order, 4-wide SSE groups, rcpps reciprocals),
+    one thread per host core, on a bounded sample of the C1 frame; the oracle's scalar
+    build is timed beside it. Parity: the host tree against the oracle's build, the timed
+    frame's hits bit for bit against the oracle's single-ray traversal, the prim ids
+    against the reference's 64-ray packets (differences only in the tie classes H1/H2,
+    counted) and the SSE build's packets against the scalar build's (H3: decisions the
+    rcpps low bits move, counted)."""
     sys.path.insert(0, str(REPO / "oracle"))
     import pyoracle
 
-    ids, nodes = scene.export_dynbvh()
     cores = host_cores()
-    total_rays = 0
-    total_s = 0.0
-    reps = {}
-    for name, rays, budget in (("primary", prim_rays, 5.0), ("bounce", bounce, 5.0)):
-        sub = {k: rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
-        n = len(sub["ox"])
-        done, spent = 0, 0.0
-        while spent < budget or done < 2:
-            t0 = time.perf_counter()
-            pyoracle.trace(verts, idx, nodes, ids, sub, packet_size=64, threads=cores)
-            spent += time.perf_counter() - t0
-            done += 1
-        reps[name] = done
-        total_rays += n * done
-        total_s += spent
-        log(f"cpu baseline {name}: {done} x {n} rays in {spent:.2f}s on {cores} threads")
-    # Thread scaling: one pass over each ray set on one thread (about a second), weighted
-    # like the sample above (the same number of passes of each set).
-    one_rays, one_s = 0, 0.0
-    for name, rays in (("primary", prim_rays), ("bounce", bounce)):
-        sub = {k: rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
-        t0 = time.perf_counter()
-        pyoracle.trace(verts, idx, nodes, ids, sub, packet_size=64, threads=1)
-        one_s += (time.perf_counter() - t0) * reps[name]
-        one_rays += len(sub["ox"]) * reps[name]
-    value = total_rays / total_s / 1e6
-    one = one_rays / one_s / 1e6
-    log(f"cpu baseline one thread: {one:.2f} Mrays/s ({value / one:.1f}x on {cores} threads)")
-    return {"value": round(value, 3), "unit": "Mrays/s", "cores": cores,
+    ids, nodes = scene.export_dynbvh()
+    t0 = time.perf_counter()
+    o_nodes, o_ids, _ = pyoracle.build_bvh(verts, idx)
+    log(f"oracle build {time.perf_counter() - t0:.1f}s")
+    parity = {"tree_equal": bool(np.array_equal(ids, o_ids) and
+                                 nodes.tobytes() == o_nodes.tobytes()),
+              "reference": "oracle single-ray traversal (bits of prim, t, u, v); 64-ray "
+                           "packets (prim ids; H1/H2 tie classes); SSE-build packets vs "
+                           "scalar-build packets (H3)"}
+    mism_total = 0
+    sets = (("primary", prim_rays), ("bounce", bounce))
+    subs = {name: {k: rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
+            for name, rays in sets}
+    for name, _ in sets:
+        sub = subs[name]
+        g = frame_hits[name]
+        o1 = pyoracle.trace(verts, idx, o_nodes, o_ids, sub, packet_size=1, threads=cores)
+        bad = g["prim"] != o1["prim"]
+        for k in ("t", "u", "v"):
+            bad |= g[k].view(np.uint32) != o1[k].view(np.uint32)
+        o64 = pyoracle.trace(verts, idx, o_nodes, o_ids, sub, packet_size=64, threads=cores)
+        h1, h2, other = tie_classes(o64["prim"], o64["t"], g["prim"], g["t"])
+        s64 = pyoracle.trace(verts, idx, o_nodes, o_ids, sub, packet_size=64, threads=cores,
+                             sse=True)
+        flips = s64["prim"] != o64["prim"]
+        f1, f2, f3 = tie_classes(o64["prim"], o64["t"], s64["prim"], s64["t"])
+        parity[name] = {"rays": int(len(g["prim"])), "mismatch": int(bad.sum()),
+                        "packets_h1": h1, "packets_h2": h2, "packets_other": other,
+                        "sse_h3_prim_flips": int(flips.sum()),
+                        "sse_h3_flips_in_tie_bands": f1 + f2,
+                        "sse_h3_t_lowbit_rays": int((s64["t"].view(np.uint32) !=
+                                                     o64["t"].view(np.uint32)).sum())}
+        mism_total += int(bad.sum()) + other
+    parity["mismatch"] = mism_total
+    log(f"parity of the timed frame: {parity}")
+
+    def rate(sse, threads, budget):
+        """Frame rate: (primary + bounce rays) / (time of one pass over each set), each
+        pass time averaged over repeated passes for about `budget` seconds per set."""
+        frame_rays, frame_s, reps = 0, 0.0, {}
+        for name, _ in sets:
+            sub = subs[name]
+            n = len(sub["ox"])
+            done, t_set = 0, 0.0
+            while t_set < budget or done < 1:
+                t0 = time.perf_counter()
+                pyoracle.trace(verts, idx, nodes, ids, sub, packet_size=64, threads=threads,
+                               sse=sse)
+                t_set += time.perf_counter() - t0
+                done += 1
+            reps[name] = done
+            frame_rays += n
+            frame_s += t_set / done
+            log(f"cpu {'sse' if sse else 'scalar'} {name}: {done} x {n} rays in {t_set:.2f}s "
+                f"on {threads} threads")
+        return frame_rays / frame_s / 1e6, reps
+
+    value, reps = rate(True, cores, 5.0)
+    one, _ = rate(True, 1, 1.0)
+    scalar, _ = rate(False, cores, 2.0)
+    log(f"cpu baseline: SSE {value:.1f} Mrays/s on {cores} threads ({one:.2f} on one), "
+        f"scalar build {scalar:.1f}")
+    base = {"value": round(value, 3), "unit": "Mrays/s", "cores": cores,
             "one_thread": round(one, 3), "thread_scaling": round(value / one, 2),
             "kind": "port",
-            "sample": f"the full frame's {len(prim_rays['ox'])} primary rays x{reps['primary']} "
+            "build": "the reference's SSE build (MANTA_SSE) restated by the oracle",
+            "scalar_build_mrays": round(scalar, 3),
+            "sample": f"the C1 frame's {len(prim_rays['ox'])} primary rays x{reps['primary']} "
                       f"and {len(bounce['ox'])} bounce rays x{reps['bounce']}, 64-ray packets "
-                      f"in 8x8-tile order, closest-hit, ~10 s of CPU time"}
+                      f"in 8x8-tile order, closest-hit, about 10 s of CPU time; the rate is "
+                      f"the frame's rays over one pass of each set"}
+    return base, parity
 
 
 if __name__ == "__main__":

@@ -24,10 +24,20 @@
 //   * DynBVH binary file layout (DynBVH.cc:2442-2501): u32 n_ids, i32 ids[], u32 n_nodes,
 //     32-byte nodes.
 //
+//   * The SSE build of the packet path (MANTA_SSE, the reference's x86 default), as a
+//     second mode for the CPU baseline and for classifying the rcpps hazard:
+//     RayPacket::actualComputeInverseDirections SSE branch (Interface/RayPacket.cc:80-104)
+//     with oneOver (Core/Math/SSEDefs.h:193-197), the SSE interval data of
+//     DynBVH::intersect (DynBVH.cc:130-236) with min4f/max4f (SSEDefs.h:269-299),
+//     firstIntersects and lastIntersects SSE branches (DynBVH.cc:423-629, 743-879),
+//     KenslerShirleyTriangle::intersect SSE branch (KenslerShirleyTriangle.cc:134-332,
+//     cross4 :127-132, dot4 SSEDefs.h:148-154) and RayPacket::hitFinal
+//     (Interface/RayPacket.h:545-605). Kensler-Shirley only.
+//
 // Numerics: every float operation is IEEE binary32 with round-to-nearest and no
 // contraction (g++ -O2 on baseline x86-64 emits no FMA). The scalar build of the
-// reference is followed; the SSE build's rcpps-based reciprocals are CPU-model specific
-// and are not restated (see the parity hazard list in the design notes).
+// reference is the default mode; the SSE mode's rcpps reciprocals are those of the CPU it
+// runs on (Intel and AMD differ in the low bits), so its results are machine specific.
 //
 // Parity pins: tests/test_oracle_pins.py checks this file against brute force over all
 // triangles, double-precision closed forms, hand-computed known-answer rays and the BVH
@@ -42,6 +52,8 @@
 #include <cstring>
 #include <thread>
 #include <vector>
+
+#include <xmmintrin.h>
 
 namespace {
 
@@ -600,7 +612,7 @@ Cost2 subtreeCost(std::vector<Node32>& nodes, const TTP& ttp, int id) {
 // RayPacketData, reduced to the fields the path touches (Interface/RayPacket.h:58-130).
 // state: 0 = not hit (hitMatl == 0), 1 = hit (hitMatl = mesh material), -1 = masked
 // (hitMatl == ~0, RayPacket.h:488-502).
-struct Packet {
+struct alignas(16) Packet {
   float org[3][64];
   float dir[3][64];
   float inv[3][64];
@@ -833,13 +845,281 @@ int lastIntersects(const Box& box, const Packet& p, int rb, int re) {
   return rb;
 }
 
+// ---------------------------------------------------------------------------
+// SSE build (MANTA_SSE): 4-wide groups of aligned rays. _mm_min_ps(a, b) is a < b ? a : b
+// and _mm_max_ps(a, b) is a > b ? a : b (the second operand when either is NaN), which the
+// intrinsics below give exactly.
+// ---------------------------------------------------------------------------
+// oneOver (Core/Math/SSEDefs.h:193-197): rcpps and one Newton step, 2r - r*r*v.
+inline __m128 oneOver4(__m128 v) {
+  const __m128 r = _mm_rcp_ps(v);
+  return _mm_sub_ps(_mm_add_ps(r, r), _mm_mul_ps(_mm_mul_ps(r, r), v));
+}
+
+// min4f / max4f (SSEDefs.h:269-299)
+inline float min4f(__m128 t) {
+  const __m128 c = _mm_min_ps(_mm_unpacklo_ps(t, t), _mm_unpackhi_ps(t, t));
+  return _mm_cvtss_f32(_mm_min_ss(c, _mm_movehl_ps(c, c)));
+}
+inline float max4f(__m128 t) {
+  const __m128 c = _mm_max_ps(_mm_unpacklo_ps(t, t), _mm_unpackhi_ps(t, t));
+  return _mm_cvtss_f32(_mm_max_ss(c, _mm_movehl_ps(c, c)));
+}
+
+inline float lane(__m128 v, int k) {
+  alignas(16) float a[4];
+  _mm_store_ps(a, v);
+  return a[k];
+}
+
+// KenslerShirleyTriangle::intersect, SSE branch (KenslerShirleyTriangle.cc:134-332): the
+// aligned groups of [b, e) four at a time, the rest with the scalar test.
+void intersectKS_SSE(const Scene& sc, int tri, Packet& p, int b, int e) {
+  const int sb = (b + 3) & ~3, se = e & ~3;
+  if (sb >= se) {
+    intersectKS(sc, tri, p, b, e);
+    return;
+  }
+  const __m128 eps4 = _mm_set1_ps(kTEps);
+  const __m128 zero = _mm_setzero_ps();
+  const float* P0 = sc.mesh.vtx(tri, 0);
+  const float* P1 = sc.mesh.vtx(tri, 1);
+  const float* P2 = sc.mesh.vtx(tri, 2);
+  const __m128 p0x = _mm_set1_ps(P0[0]), p0y = _mm_set1_ps(P0[1]), p0z = _mm_set1_ps(P0[2]);
+  const __m128 e0x = _mm_sub_ps(_mm_set1_ps(P1[0]), p0x);
+  const __m128 e0y = _mm_sub_ps(_mm_set1_ps(P1[1]), p0y);
+  const __m128 e0z = _mm_sub_ps(_mm_set1_ps(P1[2]), p0z);
+  const __m128 e1x = _mm_sub_ps(p0x, _mm_set1_ps(P2[0]));
+  const __m128 e1y = _mm_sub_ps(p0y, _mm_set1_ps(P2[1]));
+  const __m128 e1z = _mm_sub_ps(p0z, _mm_set1_ps(P2[2]));
+  // cross4 (:127-132): x = y1 z2 - z1 y2, y = z1 x2 - x1 z2, z = x1 y2 - y1 x2
+  const __m128 nx = _mm_sub_ps(_mm_mul_ps(e0y, e1z), _mm_mul_ps(e0z, e1y));
+  const __m128 ny = _mm_sub_ps(_mm_mul_ps(e0z, e1x), _mm_mul_ps(e0x, e1z));
+  const __m128 nz = _mm_sub_ps(_mm_mul_ps(e0x, e1y), _mm_mul_ps(e0y, e1x));
+  for (int r = sb; r < se; r += 4) {
+    const __m128 ox = _mm_load_ps(&p.org[0][r]), oy = _mm_load_ps(&p.org[1][r]),
+                 oz = _mm_load_ps(&p.org[2][r]);
+    const __m128 oldt = _mm_load_ps(&p.minT[r]);
+    const __m128 dx = _mm_load_ps(&p.dir[0][r]), dy = _mm_load_ps(&p.dir[1][r]),
+                 dz = _mm_load_ps(&p.dir[2][r]);
+    // dot4(o, v) (SSEDefs.h:148-154) = (vx ox + vy oy) + vz oz
+    const __m128 det = _mm_add_ps(_mm_add_ps(_mm_mul_ps(dx, nx), _mm_mul_ps(dy, ny)),
+                                  _mm_mul_ps(dz, nz));
+    const __m128 rcp = oneOver4(det);
+    const __m128 g0 = _mm_sub_ps(p0x, ox), g1 = _mm_sub_ps(p0y, oy), g2 = _mm_sub_ps(p0z, oz);
+    const __m128 t = _mm_add_ps(_mm_add_ps(_mm_mul_ps(g0, nx), _mm_mul_ps(g1, ny)),
+                                _mm_mul_ps(g2, nz));
+    const __m128 toverd = _mm_mul_ps(rcp, t);
+    __m128 mask = _mm_and_ps(_mm_cmpgt_ps(toverd, eps4),
+                             _mm_cmplt_ps(toverd, _mm_sub_ps(oldt, eps4)));
+    if (_mm_movemask_ps(mask) == 0) continue;
+    const __m128 ix = _mm_sub_ps(_mm_mul_ps(g1, dz), _mm_mul_ps(g2, dy));
+    const __m128 iy = _mm_sub_ps(_mm_mul_ps(g2, dx), _mm_mul_ps(g0, dz));
+    const __m128 iz = _mm_sub_ps(_mm_mul_ps(g0, dy), _mm_mul_ps(g1, dx));
+    const __m128 u = _mm_add_ps(_mm_add_ps(_mm_mul_ps(e1x, ix), _mm_mul_ps(e1y, iy)),
+                                _mm_mul_ps(e1z, iz));
+    const __m128 uoverd = _mm_mul_ps(rcp, u);
+    mask = _mm_and_ps(mask, _mm_cmpge_ps(uoverd, zero));
+    if (_mm_movemask_ps(mask) == 0) continue;
+    const __m128 v = _mm_add_ps(_mm_add_ps(_mm_mul_ps(e0x, ix), _mm_mul_ps(e0y, iy)),
+                                _mm_mul_ps(e0z, iz));
+    // u + v <= 1 tested as (u' + v') det <= det^2 on the undivided values (:250)
+    const __m128 uvmask = _mm_cmple_ps(_mm_mul_ps(_mm_add_ps(u, v), det), _mm_mul_ps(det, det));
+    const __m128 voverd = _mm_mul_ps(rcp, v);
+    mask = _mm_and_ps(mask, uvmask);
+    mask = _mm_and_ps(mask, _mm_cmpge_ps(voverd, zero));
+    const int m = _mm_movemask_ps(mask);
+    if (m == 0) continue;
+    // hitWithoutTminCheck -> hitFinal (RayPacket.h:545-605): t < minT as well; the
+    // scratchpad u, v are stored under the triangle's mask.
+    const int hit = _mm_movemask_ps(_mm_and_ps(mask, _mm_cmplt_ps(toverd, _mm_load_ps(&p.minT[r]))));
+    for (int k = 0; k < 4; ++k) {
+      if (hit & (1 << k)) {
+        p.minT[r + k] = lane(toverd, k);
+        p.state[r + k] = 1;
+        p.prim[r + k] = tri;
+      }
+      if (m & (1 << k)) {
+        p.u[r + k] = lane(uoverd, k);
+        p.v[r + k] = lane(voverd, k);
+      }
+    }
+  }
+  if (b < sb) intersectKS(sc, tri, p, b, sb);
+  if (se < e) intersectKS(sc, tri, p, se, e);
+}
+
+// The one-ray SSE slab test of firstIntersects (DynBVH.cc:431-456, 458-483): _ss forms.
+inline bool slabSS(const Box& box, const Packet& p, int i) {
+  auto mn = [](float a, float b) { return a < b ? a : b; };  // _mm_min_ss
+  auto mx = [](float a, float b) { return a > b ? a : b; };  // _mm_max_ss
+  float tmin = kTEps, tmax = p.minT[i];
+  for (int c = 0; c < 3; ++c) {
+    const float t0 = (box.lo[c] - p.org[c][i]) * p.inv[c][i];
+    const float t1 = (box.hi[c] - p.org[c][i]) * p.inv[c][i];
+    tmin = mx(mn(t0, t1), tmin);
+    tmax = mn(mx(t0, t1), tmax);
+  }
+  return tmin <= tmax;
+}
+
+// One aligned group of four rays (DynBVH.cc:583-604 / 817-845): the rays whose slab test
+// passes, as a movemask.
+inline int slab4(const Box& box, const Packet& p, int i) {
+  const __m128 x0 = _mm_mul_ps(_mm_sub_ps(_mm_set1_ps(box.lo[0]), _mm_load_ps(&p.org[0][i])),
+                               _mm_load_ps(&p.inv[0][i]));
+  const __m128 x1 = _mm_mul_ps(_mm_sub_ps(_mm_set1_ps(box.hi[0]), _mm_load_ps(&p.org[0][i])),
+                               _mm_load_ps(&p.inv[0][i]));
+  const __m128 y0 = _mm_mul_ps(_mm_sub_ps(_mm_set1_ps(box.lo[1]), _mm_load_ps(&p.org[1][i])),
+                               _mm_load_ps(&p.inv[1][i]));
+  const __m128 y1 = _mm_mul_ps(_mm_sub_ps(_mm_set1_ps(box.hi[1]), _mm_load_ps(&p.org[1][i])),
+                               _mm_load_ps(&p.inv[1][i]));
+  const __m128 z0 = _mm_mul_ps(_mm_sub_ps(_mm_set1_ps(box.lo[2]), _mm_load_ps(&p.org[2][i])),
+                               _mm_load_ps(&p.inv[2][i]));
+  const __m128 z1 = _mm_mul_ps(_mm_sub_ps(_mm_set1_ps(box.hi[2]), _mm_load_ps(&p.org[2][i])),
+                               _mm_load_ps(&p.inv[2][i]));
+  const __m128 mxmn = _mm_max_ps(_mm_min_ps(x0, x1),
+                                 _mm_max_ps(_mm_min_ps(y0, y1),
+                                            _mm_max_ps(_mm_min_ps(z0, z1), _mm_set1_ps(kTEps))));
+  const __m128 mnmx = _mm_min_ps(_mm_max_ps(x0, x1),
+                                 _mm_min_ps(_mm_max_ps(y0, y1),
+                                            _mm_min_ps(_mm_max_ps(z0, z1),
+                                                       _mm_load_ps(&p.minT[i]))));
+  return _mm_movemask_ps(_mm_cmple_ps(mxmn, mnmx));
+}
+
+inline bool groupMasked(const Packet& p, int i) {
+  return masked(p, i) && masked(p, i + 1) && masked(p, i + 2) && masked(p, i + 3);
+}
+// RayPacket::wereNotHitSSE: hitMatl == 0 (masked rays count as hit)
+inline int notHit4(const Packet& p, int i) {
+  int m = 0;
+  for (int k = 0; k < 4; ++k) m |= (p.state[i + k] == 0) << k;
+  return m;
+}
+
+// DynBVH::firstIntersects, SSE branch (DynBVH.cc:407-629)
+int firstIntersectsSSE(const Box& box, const Packet& p, int rb, int re, const IA& ia) {
+  int rayBegin = rb;
+  const int rayEnd = re;
+  for (; rayBegin < rayEnd; rayBegin++) {
+    if (masked(p, rayBegin)) continue;
+    if (p.anyHit && wasHit(p, rayBegin)) continue;
+    break;
+  }
+  const int b = (rayBegin + 3) & ~3, e = rayEnd & ~3;
+  if (b >= e) {
+    for (int i = rayBegin; i < rayEnd; i++) {
+      if (masked(p, i)) continue;
+      if (p.anyHit && wasHit(p, i)) continue;
+      if (slabSS(box, p, i)) return i;
+    }
+    return rayEnd;
+  }
+  int i = rayBegin;
+  if (slabSS(box, p, i)) return i;
+  {  // frustum miss (the SSE branch starts from T_EPSILON)
+    float tmin_f = kTEps, tmax_f = FLT_MAX;
+    for (int axis = 0; axis < 3; ++axis) {
+      float a = box.lo[axis] - ia.max_org[axis];
+      float bb = box.hi[axis] - ia.min_org[axis];
+      const float ar0 = a * ia.min_rcp[axis], ar1 = a * ia.max_rcp[axis];
+      const float br0 = bb * ia.min_rcp[axis], br1 = bb * ia.max_rcp[axis];
+      a = (br0 < br1) ? br0 : br1;
+      a = (a < ar1) ? a : ar1;
+      a = (a < ar0) ? a : ar0;
+      bb = (br0 < br1) ? br1 : br0;
+      bb = (bb < ar1) ? ar1 : bb;
+      bb = (bb < ar0) ? ar0 : bb;
+      tmin_f = (tmin_f < a) ? a : tmin_f;
+      tmax_f = (tmax_f > bb) ? bb : tmax_f;
+    }
+    if (tmin_f > tmax_f) return rayEnd;
+  }
+  if (i < b) i++;
+  for (; i < b; i++) {
+    if (masked(p, i)) continue;
+    if (p.anyHit && wasHit(p, i)) continue;
+    if (slab(box, p, i)) return i;
+  }
+  for (; i < e; i += 4) {
+    if (groupMasked(p, i)) continue;
+    int valid = 15;
+    if (p.anyHit) {
+      valid = notHit4(p, i);
+      if (valid == 0) continue;
+    }
+    const int mask = valid & slab4(box, p, i);
+    if (mask) {
+      for (int k = 0; k < 4; ++k)
+        if ((mask & (1 << k)) && !masked(p, i + k)) return i + k;
+    }
+  }
+  for (; i < rayEnd; i++) {
+    if (masked(p, i)) continue;
+    if (p.anyHit && wasHit(p, i)) continue;
+    if (slab(box, p, i)) return i;
+  }
+  return rayEnd;
+}
+
+// DynBVH::lastIntersects, SSE branch (DynBVH.cc:728-879)
+int lastIntersectsSSE(const Box& box, const Packet& p, int rb, int re) {
+  const int rayBegin = rb;
+  int rayEnd = re;
+  for (; rayBegin < rayEnd; rayEnd--) {
+    if (masked(p, rayEnd - 1)) continue;
+    if (p.anyHit && wasHit(p, rayEnd - 1)) continue;
+    break;
+  }
+  const int last_simd = rayEnd & ~3, first_simd = (rayBegin + 3) & ~3;
+  if (first_simd >= last_simd) {
+    for (int i = rayEnd - 1; i > rayBegin; i--) {
+      if (masked(p, i)) continue;
+      if (p.anyHit && wasHit(p, i)) continue;
+      if (slab(box, p, i)) return i;
+    }
+    return rayBegin;
+  }
+  int i = rayEnd;
+  while (i > last_simd) {
+    i--;
+    if (masked(p, i)) continue;
+    if (p.anyHit && wasHit(p, i)) continue;
+    if (slab(box, p, i)) return i;
+  }
+  while (i > first_simd) {
+    i -= 4;
+    if (groupMasked(p, i)) continue;
+    int valid = 15;
+    if (p.anyHit) {
+      valid = notHit4(p, i);
+      if (valid == 0) continue;
+    }
+    const int mask = valid & slab4(box, p, i);
+    if (mask) {
+      for (int k = 3; k >= 0; --k)
+        if ((mask & (1 << k)) && !masked(p, i + k)) return i + k;
+    }
+  }
+  while (i > rayBegin) {
+    i--;
+    if (p.anyHit && wasHit(p, i)) continue;
+    if (slab(box, p, i)) return i;
+  }
+  return rayBegin;
+}
+
+template <bool SSE>
 void intersectNode(const Scene& sc, int nodeID, Packet& p, int rb, int re, const IA& ia,
                    Counters* cnt) {
   const Node32& node = sc.nodes[nodeID];
-  int firstActive = firstIntersects(node.bounds, p, rb, re, ia, cnt);
+  int firstActive = SSE ? firstIntersectsSSE(node.bounds, p, rb, re, ia)
+                        : firstIntersects(node.bounds, p, rb, re, ia, cnt);
   if (firstActive == re) return;
   if (node.isLeaf()) {
-    int lastActive = lastIntersects(node.bounds, p, rb, re);
+    int lastActive = SSE ? lastIntersectsSSE(node.bounds, p, rb, re)
+                         : lastIntersects(node.bounds, p, rb, re);
     int sb = firstActive, se = lastActive + 1;
     for (int k = 0; k < node.children; ++k) {
       const int object_id = sc.ids[node.child + k];
@@ -847,7 +1127,10 @@ void intersectNode(const Scene& sc, int nodeID, Packet& p, int rb, int re, const
       if (p.anyHit) {
         // DynBVH.cc:315-375; every material is opaque, so attenuateShadows turns each
         // newly hit ray black and the ray's minT is overridden with T_EPSILON.
-        intersectPrim(sc, object_id, p, sb, se);
+        if (SSE)
+          intersectKS_SSE(sc, object_id, p, sb, se);
+        else
+          intersectPrim(sc, object_id, p, sb, se);
         bool terminated = false;
         for (int i = sb; i < se; ++i) {
           if (!wasHit(p, i) || masked(p, i)) continue;
@@ -864,6 +1147,8 @@ void intersectNode(const Scene& sc, int nodeID, Packet& p, int rb, int re, const
           se = end + 1;
           if (end < start) break;
         }
+      } else if (SSE) {
+        intersectKS_SSE(sc, object_id, p, sb, se);
       } else {
         intersectPrim(sc, object_id, p, sb, se);
       }
@@ -874,8 +1159,18 @@ void intersectNode(const Scene& sc, int nodeID, Packet& p, int rb, int re, const
       front = node.leftCheaper() ? 0 : 1;
     else
       front = p.dir[node.axis][firstActive] > 0 ? 0 : 1;
-    intersectNode(sc, node.child + front, p, firstActive, re, ia, cnt);
-    intersectNode(sc, node.child + 1 - front, p, firstActive, re, ia, cnt);
+    intersectNode<SSE>(sc, node.child + front, p, firstActive, re, ia, cnt);
+    intersectNode<SSE>(sc, node.child + 1 - front, p, firstActive, re, ia, cnt);
+  }
+}
+
+inline void iaScalar(IA& ia, const Packet& p, int r) {
+  for (int a = 0; a < 3; ++a) {
+    const float rc = p.inv[a][r], o = p.org[a][r];
+    ia.min_rcp[a] = (ia.min_rcp[a] < rc) ? ia.min_rcp[a] : rc;
+    ia.max_rcp[a] = (ia.max_rcp[a] < rc) ? rc : ia.max_rcp[a];
+    ia.min_org[a] = (ia.min_org[a] < o) ? ia.min_org[a] : o;
+    ia.max_org[a] = (ia.max_org[a] < o) ? o : ia.max_org[a];
   }
 }
 
@@ -892,15 +1187,56 @@ void intersectPacket(const Scene& sc, Packet& p, int n, Counters* cnt) {
   }
   for (int r = 0; r < n; ++r) {
     if (masked(p, r)) continue;
-    for (int a = 0; a < 3; ++a) {
-      const float rc = p.inv[a][r], o = p.org[a][r];
-      ia.min_rcp[a] = (ia.min_rcp[a] < rc) ? ia.min_rcp[a] : rc;
-      ia.max_rcp[a] = (ia.max_rcp[a] < rc) ? rc : ia.max_rcp[a];
-      ia.min_org[a] = (ia.min_org[a] < o) ? ia.min_org[a] : o;
-      ia.max_org[a] = (ia.max_org[a] < o) ? o : ia.max_org[a];
-    }
+    iaScalar(ia, p, r);
   }
-  intersectNode(sc, 0, p, 0, n, ia, cnt);
+  intersectNode<false>(sc, 0, p, 0, n, ia, cnt);
+}
+
+// DynBVH::intersect, SSE branch: inverse directions by oneOver on the aligned groups
+// (RayPacket.cc:80-104), interval data by 4-wide min/max over the aligned groups, masked
+// rays included (DynBVH.cc:130-236), then the SSE recursion.
+void intersectPacketSSE(const Scene& sc, Packet& p, int n) {
+  const int e = n & ~3;
+  if (e <= 0) {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < 3; ++j) p.inv[j][i] = (float)1 / p.dir[j][i];
+  } else {
+    for (int i = 0; i < e; i += 4)
+      for (int j = 0; j < 3; ++j)
+        _mm_store_ps(&p.inv[j][i], oneOver4(_mm_load_ps(&p.dir[j][i])));
+    for (int i = e; i < n; ++i)
+      for (int j = 0; j < 3; ++j) p.inv[j][i] = (float)1 / p.dir[j][i];
+  }
+  IA ia;
+  for (int a = 0; a < 3; ++a) {
+    ia.min_rcp[a] = FLT_MAX;
+    ia.max_rcp[a] = -FLT_MAX;
+    ia.min_org[a] = FLT_MAX;
+    ia.max_org[a] = -FLT_MAX;
+  }
+  if (e <= 0) {
+    for (int r = 0; r < n; ++r)
+      if (!masked(p, r)) iaScalar(ia, p, r);
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      __m128 mnr = _mm_set1_ps(ia.min_rcp[a]), mxr = _mm_set1_ps(ia.max_rcp[a]);
+      __m128 mno = _mm_set1_ps(ia.min_org[a]), mxo = _mm_set1_ps(ia.max_org[a]);
+      for (int r = 0; r < e; r += 4) {
+        const __m128 rc = _mm_load_ps(&p.inv[a][r]), o = _mm_load_ps(&p.org[a][r]);
+        mnr = _mm_min_ps(mnr, rc);
+        mxr = _mm_max_ps(mxr, rc);
+        mno = _mm_min_ps(mno, o);
+        mxo = _mm_max_ps(mxo, o);
+      }
+      ia.min_rcp[a] = min4f(mnr);
+      ia.max_rcp[a] = max4f(mxr);
+      ia.min_org[a] = min4f(mno);
+      ia.max_org[a] = max4f(mxo);
+    }
+    for (int r = e; r < n; ++r)
+      if (!masked(p, r)) iaScalar(ia, p, r);
+  }
+  intersectNode<true>(sc, 0, p, 0, n, ia, nullptr);
 }
 
 struct RayIO {
@@ -912,7 +1248,7 @@ struct RayIO {
 };
 
 void tracePacketRange(const Scene& sc, const RayIO& io, int64_t first, int n, bool anyHit,
-                      Counters* cnt) {
+                      Counters* cnt, bool sse = false) {
   Packet p;
   p.anyHit = anyHit;
   for (int i = 0; i < n; ++i) {
@@ -933,7 +1269,10 @@ void tracePacketRange(const Scene& sc, const RayIO& io, int64_t first, int n, bo
       p.minT[i] = io.tmax[r];
     }
   }
-  intersectPacket(sc, p, n, cnt);
+  if (sse)
+    intersectPacketSSE(sc, p, n);
+  else
+    intersectPacket(sc, p, n, cnt);
   for (int i = 0; i < n; ++i) {
     const int64_t r = first + i;
     if (io.prim) io.prim[r] = (p.state[i] == 1) ? p.prim[i] : (p.state[i] == -1 ? -2 : -1);
@@ -1579,13 +1918,17 @@ void oracle_rtsah_costs(void* nodes, int n_nodes, float* costs4) {
 // Traces n rays through the tree in packets of packet_size (1..64) consecutive rays.
 // prim: triangle id, -1 no hit, -2 masked. t: final minT. occluded: RayPacket::wasHit.
 // counters (optional, int64[2]): total box tests and triangle tests.
-int oracle_trace(const float* verts, int nverts, const uint32_t* idx, int ntris,
-                 const void* nodes, const int32_t* ids, int prim_kind, const float* ox,
-                 const float* oy, const float* oz, const float* dx, const float* dy,
-                 const float* dz, const float* tmax, const uint8_t* maskd, int64_t n,
-                 int packet_size, int any_hit, int nthreads, int32_t* prim, float* t,
-                 float* u, float* v, uint8_t* occluded, int64_t* counters) {
+// mode 0: the scalar build (default); mode 1: the SSE build (Kensler-Shirley only, no
+// counters).
+int oracle_trace_mode(const float* verts, int nverts, const uint32_t* idx, int ntris,
+                      const void* nodes, const int32_t* ids, int prim_kind, const float* ox,
+                      const float* oy, const float* oz, const float* dx, const float* dy,
+                      const float* dz, const float* tmax, const uint8_t* maskd, int64_t n,
+                      int packet_size, int any_hit, int nthreads, int32_t* prim, float* t,
+                      float* u, float* v, uint8_t* occluded, int64_t* counters, int mode) {
   if (packet_size < 1 || packet_size > 64) return -1;
+  if (mode != 0 && mode != 1) return -1;
+  if (mode == 1 && (prim_kind != 0 || counters)) return -1;
   Scene sc;
   sc.mesh = Mesh{verts, nverts, idx, ntris};
   sc.nodes = (const Node32*)nodes;
@@ -1603,7 +1946,8 @@ int oracle_trace(const float* verts, int nverts, const uint32_t* idx, int ntris,
   parallelFor(npk, nt, [&](int64_t k, int tid) {
     int64_t first = k * packet_size;
     int cnt = (int)std::min<int64_t>(packet_size, n - first);
-    tracePacketRange(sc, io, first, cnt, any_hit != 0, counters ? &per[tid] : nullptr);
+    tracePacketRange(sc, io, first, cnt, any_hit != 0, counters ? &per[tid] : nullptr,
+                     mode == 1);
   });
   if (counters) {
     counters[0] = counters[1] = 0;
@@ -1613,6 +1957,17 @@ int oracle_trace(const float* verts, int nverts, const uint32_t* idx, int ntris,
     }
   }
   return 0;
+}
+
+int oracle_trace(const float* verts, int nverts, const uint32_t* idx, int ntris,
+                 const void* nodes, const int32_t* ids, int prim_kind, const float* ox,
+                 const float* oy, const float* oz, const float* dx, const float* dy,
+                 const float* dz, const float* tmax, const uint8_t* maskd, int64_t n,
+                 int packet_size, int any_hit, int nthreads, int32_t* prim, float* t,
+                 float* u, float* v, uint8_t* occluded, int64_t* counters) {
+  return oracle_trace_mode(verts, nverts, idx, ntris, nodes, ids, prim_kind, ox, oy, oz, dx,
+                           dy, dz, tmax, maskd, n, packet_size, any_hit, nthreads, prim, t, u,
+                           v, occluded, counters, 0);
 }
 
 // Brute force over all triangles in ascending id order with the same triangle test and

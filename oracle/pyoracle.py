@@ -50,6 +50,8 @@ def load() -> C.CDLL:
         lib.oracle_trace.restype = C.c_int
         lib.oracle_trace.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_int] + [P] * 8 + [
             C.c_int64, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P]
+        lib.oracle_trace_mode.restype = C.c_int
+        lib.oracle_trace_mode.argtypes = lib.oracle_trace.argtypes + [C.c_int]
         lib.oracle_bruteforce.restype = C.c_int
         lib.oracle_bruteforce.argtypes = [P, C.c_int, P, C.c_int, C.c_int] + [P] * 8 + [
             C.c_int64, C.c_int, C.c_int, P, P, P, P, P]
@@ -152,8 +154,9 @@ def _rays(rays):
 
 
 def trace(verts, idx, nodes, ids, rays, any_hit=False, packet_size=1, prim_kind=0,
-          threads=None, counters=False):
-    """Packet traversal over consecutive groups of packet_size rays."""
+          threads=None, counters=False, sse=False):
+    """Packet traversal over consecutive groups of packet_size rays. sse=True runs the
+    reference's SSE build (rcpps reciprocals of this CPU; Kensler-Shirley only)."""
     v, i = _mesh(verts, idx)
     arr, m = _rays(rays)
     n = len(arr["ox"])
@@ -162,12 +165,13 @@ def trace(verts, idx, nodes, ids, rays, any_hit=False, packet_size=1, prim_kind=
            "occluded": np.empty(n, np.uint8)}
     cnt = np.zeros(2, np.int64) if counters else None
     th = threads if threads is not None else (os.cpu_count() or 1)
-    rc = load().oracle_trace(_p(v), len(v), _p(i), len(i), _p(np.ascontiguousarray(nodes)),
-                             _p(np.ascontiguousarray(ids, np.int32)), prim_kind,
-                             *[_p(arr[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")],
-                             _p(m), n, packet_size, int(any_hit), th, _p(out["prim"]),
-                             _p(out["t"]), _p(out["u"]), _p(out["v"]), _p(out["occluded"]),
-                             _p(cnt))
+    rc = load().oracle_trace_mode(_p(v), len(v), _p(i), len(i), _p(np.ascontiguousarray(nodes)),
+                                  _p(np.ascontiguousarray(ids, np.int32)), prim_kind,
+                                  *[_p(arr[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz",
+                                                         "tmax")],
+                                  _p(m), n, packet_size, int(any_hit), th, _p(out["prim"]),
+                                  _p(out["t"]), _p(out["u"]), _p(out["v"]), _p(out["occluded"]),
+                                  _p(cnt), int(bool(sse)))
     if rc != 0:
         raise ValueError("oracle trace failed")
     if counters:

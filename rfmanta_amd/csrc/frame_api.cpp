@@ -33,6 +33,9 @@ struct mh_frame {
   int64_t spill_stride = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // The scene's device view as the captured launch holds it (kernel arguments are
+  // frozen by value): buffer addresses and bounds, root reference, triangle test.
+  mh::DevScene captured{};
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -160,6 +163,20 @@ int launch(mh_frame* f, void* stream) {
   bl.xcd_remap = f->cfg.xcd_remap;
   bl.kinds = kinds_of(f->nb, f->n_any);
   HIP_OK((hipError_t)mh::launch_trace_batch(f->scene->dev(), bl, stream));
+  return MH_OK;
+}
+
+// A captured graph replays the launch exactly as it was recorded: refuse it when the
+// scene's device view changed since (a rebuild, a load, another triangle test, buffers
+// that moved) or when the tree deepened past the spill area it was given.
+int graph_current(const mh_frame* f) {
+  if (!f->exec) return fail(MH_E_STATE, "no captured graph for the current configuration");
+  const mh::DevScene now = f->scene->dev();
+  if (std::memcmp(&now, &f->captured, sizeof(now)) != 0)
+    return fail(MH_E_STATE, "the scene was rebuilt or reloaded since the capture: capture "
+                "the frame again");
+  if (!spill_ok(f))
+    return fail(MH_E_STATE, "the tree deepened since the capture: capture the frame again");
   return MH_OK;
 }
 
@@ -331,23 +348,23 @@ int mh_frame_capture(mh_frame* f, void* stream) {
   if (e != hipSuccess) return fail(MH_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
   f->graph = g;
   HIP_OK(hipGraphInstantiate(&f->exec, g, nullptr, nullptr, 0));
+  f->captured = f->scene->dev();
   return MH_OK;
 }
 
 int mh_frame_replay(mh_frame* f, void* stream) {
   if (!f) return fail(MH_E_INVALID, "null frame");
-  if (!f->exec) return fail(MH_E_STATE, "no captured graph for the current configuration");
-  if (!spill_ok(f))
-    return fail(MH_E_STATE, "the tree deepened since the capture: capture the frame again");
+  int rc = graph_current(f);
+  if (rc) return rc;
   HIP_OK(hipGraphLaunch(f->exec, reinterpret_cast<hipStream_t>(stream)));
   return MH_OK;
 }
 
 int mh_frame_time(mh_frame* f, int iters, int use_graph, void* stream, float* ms) {
   if (!f || !ms || iters < 1) return fail(MH_E_INVALID, "bad arguments");
-  if (use_graph && !f->exec) return fail(MH_E_STATE, "no captured graph");
   int rc = ensure_init();
   if (rc) return rc;
+  if (use_graph && (rc = graph_current(f))) return rc;
   return time_frame(f, iters, use_graph != 0, stream, ms);
 }
 

@@ -113,7 +113,7 @@ struct mh_scene {
     return depth;
   }
   mh::DevScene dev() const {
-    mh::DevScene d;
+    mh::DevScene d{};
     d.pairs = pairs.as<const mh::PairNode>();
     d.refs = refs.as<const mh::RefNode>();
     d.tris = tris.as<const float>();

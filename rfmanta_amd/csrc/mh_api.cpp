@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -121,8 +122,10 @@ void rotation_state(const std::vector<mh::RefNode>& nodes, std::vector<float>& c
 
 // Lays the reference tree out for the device: PairNodes indexed by sibling pair, leaf
 // references, the leaf and per-depth lists for refits, the rotation state, then uploads.
-int upload(mh_scene* s) {
-  const std::vector<mh::RefNode>& nodes = s->bvh.nodes;
+// With rtsah, isLeftCheaper is recomputed once the records are known to form a tree
+// (DynBVH::buildFromFile runs computeTraversalCost after reading, DynBVH.cc:2469-2472).
+int upload(mh_scene* s, bool rtsah = false) {
+  std::vector<mh::RefNode>& nodes = s->bvh.nodes;
   const uint32_t nrec = (uint32_t)nodes.size();
   const uint32_t T = s->nt;
   std::vector<int32_t> leaves;
@@ -169,6 +172,7 @@ int upload(mh_scene* s) {
     st.push_back({n.child + 1, it.depth + 1});
     st.push_back({n.child, it.depth + 1});
   }
+  if (rtsah) mh::compute_rtsah(nodes, 0);
   auto encode = [&](int32_t rec) -> int32_t {
     const mh::RefNode& n = nodes[rec];
     if (!n.leaf()) return (n.child - 1) >> 1;
@@ -227,8 +231,12 @@ int upload(mh_scene* s) {
   if ((rc = s->tris.reserve((size_t)T * 12 * sizeof(float)))) return rc;
   if ((rc = s->ids.reserve((size_t)T * sizeof(int32_t)))) return rc;
   if ((rc = s->big.reserve((size_t)T * sizeof(int32_t)))) return rc;
-  if ((rc = s->leaves.reserve((size_t)std::max(1u, T) * sizeof(int32_t)))) return rc;
-  if ((rc = s->levels.reserve((size_t)std::max(1u, T) * sizeof(int32_t)))) return rc;
+  // Leaf and level lists: room for T entries, which the rotating update needs (a
+  // partition of T objects has at most T leaves and T - 1 inner records), and for every
+  // record of a loaded tree whose leaves overlap (it can hold more leaves than objects).
+  const size_t list_cap = std::max<size_t>({(size_t)std::max(1u, T), leaves.size(), levels.size()});
+  if ((rc = s->leaves.reserve(list_cap * sizeof(int32_t)))) return rc;
+  if ((rc = s->levels.reserve(list_cap * sizeof(int32_t)))) return rc;
   if ((rc = s->costs.reserve(nrec * sizeof(float)))) return rc;
   if ((rc = s->ssize.reserve(nrec * sizeof(uint32_t)))) return rc;
   HIP_OK(hipMemcpy(s->refs.p, nodes.data(), nrec * sizeof(mh::RefNode), hipMemcpyHostToDevice));
@@ -640,8 +648,10 @@ int mh_dynbvh_build_host(const float* verts, uint32_t n_verts, const uint32_t* i
   return MH_OK;
 }
 
-int mh_scene_load_dynbvh(mh_scene* s, const int32_t* ids, uint32_t n_ids, const void* nodes,
-                         uint32_t n_nodes, int prim_kind) {
+namespace {
+
+int load_tree(mh_scene* s, const int32_t* ids, uint32_t n_ids, const void* nodes,
+              uint32_t n_nodes, int prim_kind, bool rtsah) {
   if (!s || !ids || !nodes) return fail(MH_E_INVALID, "null argument");
   if (prim_kind != MH_PRIM_KENSLER_SHIRLEY && prim_kind != MH_PRIM_WALD)
     return fail(MH_E_INVALID, "unknown prim_kind %d", prim_kind);
@@ -656,7 +666,14 @@ int mh_scene_load_dynbvh(mh_scene* s, const int32_t* ids, uint32_t n_ids, const 
   s->bvh.num_nodes = 0;
   s->prim_kind = prim_kind;
   s->built = false;
-  return upload(s);
+  return upload(s, rtsah);
+}
+
+}  // namespace
+
+int mh_scene_load_dynbvh(mh_scene* s, const int32_t* ids, uint32_t n_ids, const void* nodes,
+                         uint32_t n_nodes, int prim_kind) {
+  return load_tree(s, ids, n_ids, nodes, n_nodes, prim_kind, false);
 }
 
 int mh_scene_load_dynbvh_file(mh_scene* s, const char* path, int prim_kind) {
@@ -673,7 +690,7 @@ int mh_scene_load_dynbvh_file(mh_scene* s, const char* path, int prim_kind) {
   std::vector<mh::RefNode> nodes(n_nodes);
   in.read(reinterpret_cast<char*>(nodes.data()), (std::streamsize)n_nodes * 32);
   if (!in) return fail(MH_E_IO, "%s: truncated", path);
-  return mh_scene_load_dynbvh(s, ids.data(), n_ids, nodes.data(), n_nodes, prim_kind);
+  return load_tree(s, ids.data(), n_ids, nodes.data(), n_nodes, prim_kind, true);
 }
 
 int mh_scene_export_dynbvh(const mh_scene* s, int32_t* ids, uint32_t* n_ids, void* nodes,
@@ -844,6 +861,12 @@ int mh_scene_update(mh_scene* s, void* stream) {
   // tree grew deeper than that, the pass says so and runs again with all eight steps.
   int steps = 2;
   while (steps < 8 && (1u << steps) <= s->depth + 8) steps += 2;
+  // MH_UPDATE_JUMP_STEPS (2, 4, 6 or 8) forces the first pass's step count, so that tests
+  // can reach the re-run with a tree deeper than the first pass covers.
+  if (const char* e = std::getenv("MH_UPDATE_JUMP_STEPS")) {
+    const int v = std::atoi(e);
+    if (v == 2 || v == 4 || v == 6 || v == 8) steps = v;
+  }
   std::vector<unsigned> hist(mh::kDepthHist);
   unsigned back[2];
   for (;;) {

@@ -345,3 +345,33 @@ def config2_batches(verts, idx, n_batches: int = 64, lo: int = 1_000, hi: int = 
         out.append((str(kinds[b]), batch_rays(str(kinds[b]), int(sizes[b]), verts, idx, seed, start)))
         start += int(sizes[b])
     return out
+
+
+def c3_requests(n: int, seed: int = 2):
+    """Serving workload: n requests (scene, camera keyword arguments) from PCG32(seed).
+    Scene 0 is the 1M-triangle heightfield, scene 1 the 8M-triangle soup in [-4,4]^3; each
+    request has a random orbit camera at 256^2, 512^2, 1024x768 or 1920x1080 with 1-4
+    samples per pixel."""
+    r = pcg32(seed, np.arange(n, dtype=np.uint64), 6)
+    u = [u01(r[k]) for k in range(6)]
+    res = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+    out = []
+    for i in range(n):
+        sc = int(u[0][i] < 0.5)
+        xr, yr = res[min(3, int(u[1][i] * 4))]
+        spp = 1 + min(3, int(u[2][i] * 4))
+        az = 2 * np.pi * u[3][i]
+        el = 0.35 + 0.5 * u[4][i]
+        if sc == 0:  # heightfield in [-1,1] x [-0.5,0.5]
+            rad, look = 1.6, (0.0, 0.0, 0.0)
+        else:        # soup in [-4,4]^3
+            rad, look = 11.0, (0.0, 0.0, 0.0)
+        eye = (rad * np.cos(el) * np.cos(az), rad * np.sin(el), rad * np.cos(el) * np.sin(az))
+        out.append((sc, dict(eye=eye, lookat=look, up=(0, 1, 0), vfov_deg=40.0, xres=xr,
+                             yres=yr, spp=spp, seed=i)))
+    return out
+
+
+C3_LIGHT = (2.0, 6.0, -2.0)
+C4_CAMERA = dict(eye=(0.0, 0.7, -0.9), lookat=(0.0, 0.0, 0.0), up=(0, 1, 0), vfov_deg=30.0,
+                 xres=1920, yres=1080, seed=4)

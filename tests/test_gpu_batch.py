@@ -182,3 +182,38 @@ def test_frame_wald_matches_oracle(oracle, small, cfg):
                 assert np.array_equal(_bits(g["u"]), _bits(o["u"]))
                 assert np.array_equal(_bits(g["v"]), _bits(o["v"]))
         f.close()
+
+
+def test_graph_refused_after_rebuild(small):
+    """A captured frame holds the scene's device view by value: after a rebuild with the
+    Wald test (another triangle kernel, new buffers) replaying it is refused until the
+    frame is captured again, and the new capture replays correctly."""
+    from rfmanta_amd import _lib as L
+    v, i, batches = small
+    with api.Scene(v, i) as s:
+        s.build()
+        dev = _device(batches)
+        stream = api.stream_create()
+        try:
+            f = api.Frame(s, dev)
+            f.capture(stream)
+            f.replay(stream)
+            api.sync(stream)
+            s.build(prim_kind=L.MH_PRIM_WALD)
+            with pytest.raises(L.MhError, match="capture the frame again"):
+                f.replay(stream)
+            with pytest.raises(L.MhError, match="capture the frame again"):
+                f.time(1, use_graph=True, stream=stream)
+            f.launch()  # an eager launch reads the scene as it is now
+            api.sync()
+            ref = [dh.download() for _, dh, _ in dev]
+            f.capture(stream)
+            for _, dh, _ in dev:
+                dh.arrays["prim"].upload(np.full(dh.n, 777, np.int32))
+            f.replay(stream)
+            api.sync(stream)
+            for (_, dh, _), r in zip(dev, ref):
+                assert np.array_equal(dh.download()["prim"], r["prim"])
+            f.close()
+        finally:
+            api.stream_destroy(stream)

@@ -201,3 +201,118 @@ def test_single_triangle_and_tiny_meshes(oracle):
                 g = s.trace_host(rays, packet_size=packet)
                 o = oracle.trace(v, i, nodes, ids, rays, packet_size=packet)
                 assert np.array_equal(g["prim"], o["prim"]), (T, packet)
+
+
+def _write_tree(path, ids, nodes):
+    """DynBVH::saveToFile layout: u32 n_ids, i32 ids[], u32 n_nodes, 32-byte nodes."""
+    with open(path, "wb") as f:
+        f.write(np.uint32(len(ids)).tobytes())
+        f.write(np.ascontiguousarray(ids, np.int32).tobytes())
+        f.write(np.uint32(len(nodes)).tobytes())
+        f.write(np.ascontiguousarray(nodes).tobytes())
+
+
+def test_file_load_recomputes_rtsah(oracle, tmp_path):
+    """DynBVH::buildFromFile runs computeTraversalCost after reading (DynBVH.cc:2469-2472):
+    a file whose isLeftCheaper bits are cleared loads with the bits RTSAH gives, and any-hit
+    rays report the occluders the oracle's traversal of that tree finds."""
+    v, i = scenes.soup(3000, seed=21)
+    nodes, ids, _ = oracle.build_bvh(v, i)
+    want = nodes.copy()
+    oracle.rtsah(want)
+    cleared = nodes.copy()
+    cleared["bits"] &= np.uint8(0xFE)
+    assert (cleared["bits"] != want["bits"]).any()
+    p = tmp_path / "tree.bvh"
+    _write_tree(p, ids, cleared)
+    rays = scenes.random_rays(4096, 22, (-1.2, -1.2, -1.2), (1.2, 1.2, 1.2))
+    with api.Scene(v, i) as s:
+        s.load_dynbvh_file(str(p))
+        _, gn = s.export_dynbvh()
+        assert gn.tobytes() == want.tobytes()
+        g = s.trace_host(rays, any_hit=True)
+        o = oracle.trace(v, i, want, ids, rays, any_hit=True, packet_size=1)
+        assert np.array_equal(g["prim"], o["prim"])
+        assert np.array_equal(g["occluded"], o["occluded"])
+        # the in-memory import takes the bits as given
+        s.load_dynbvh(ids, cleared)
+        _, gn = s.export_dynbvh()
+        assert gn.tobytes() == cleared.tobytes()
+
+
+def test_overlapping_leaves_load_and_trace(oracle):
+    """A tree whose ten leaves each hold all four objects (more leaves than objects, so
+    not a partition): it loads, traces like the oracle and refits, and the update refuses."""
+    from rfmanta_amd import _lib as L
+    v, i = scenes.soup(4, seed=23, sigma=0.3)
+    lo, hi = v.min(0), v.max(0)
+    nodes = np.zeros(19, L.NODE_DTYPE)
+    nodes["lo"], nodes["hi"] = lo, hi
+    # records 0..8 inner (children 2k+1, 2k+2), records 9..18 leaves over slots [0, 4)
+    for k in range(9):
+        nodes["child"][k] = 2 * k + 1
+    for k in range(9, 19):
+        nodes["child"][k] = 0
+        nodes["children"][k] = 4
+    ids = np.arange(4, dtype=np.int32)
+    rays = scenes.random_rays(2048, 24, (-1.5, -1.5, -1.5), (1.5, 1.5, 1.5))
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes)
+        assert s.stats()["nodes"] == 19
+        for packet in (1, 64):
+            g = s.trace_host(rays, packet_size=packet)
+            o = oracle.trace(v, i, nodes, ids, rays, packet_size=packet)
+            assert np.array_equal(g["prim"], o["prim"]), packet
+            assert _same_bits(g["t"], o["t"]), packet
+        s.refit()
+        api.sync()
+        _, gn = s.export_dynbvh()
+        ref = nodes.copy()
+        oracle.refit(v, i, ref, ids)
+        assert gn.tobytes() == ref.tobytes()
+        with pytest.raises(L.MhError, match="refit"):
+            s.update()
+
+
+def test_non_finite_and_degenerate_rays(oracle):
+    """Directions with +-0, +-inf and NaN components, zero directions, non-finite origins
+    and tmax of inf, 0, negative and NaN: the device gives the oracle's results (the
+    reference's scalar arithmetic, select-based min/max included) at packet sizes 1 and 64,
+    closest-hit and any-hit."""
+    v, i = scenes.soup(2000, seed=31)
+    nodes, ids, _ = oracle.build_bvh(v, i)
+    n = 64 * 12
+    rays = scenes.random_rays(n, 32, (-1.2, -1.2, -1.2), (1.2, 1.2, 1.2))
+    inf, nan = np.inf, np.nan
+    dirs = [(0.0, 0.0, 1.0), (-0.0, 0.0, 1.0), (0.0, -0.0, -1.0), (inf, 0.0, 0.0),
+            (-inf, 1.0, 0.0), (nan, 0.0, 1.0), (0.0, 0.0, 0.0), (-0.0, -0.0, -0.0),
+            (1.0, inf, -inf), (nan, nan, nan), (1e-30, -1e-30, 1.0), (0.0, 1e38, 0.0)]
+    orgs = [(inf, 0.0, 0.0), (-inf, 0.1, 0.0), (nan, 0.0, 0.0), (0.0, 1e30, 0.0),
+            (0.0, 0.0, -inf)]
+    tmaxs = [np.float32(np.inf), np.float32(0.0), np.float32(-1.0), np.float32(np.nan)]
+    k = np.arange(n)
+    for j in np.nonzero(k % 3 == 0)[0]:
+        d = dirs[(j // 3) % len(dirs)]
+        rays["dx"][j], rays["dy"][j], rays["dz"][j] = d
+    for j in np.nonzero(k % 5 == 1)[0]:
+        o = orgs[(j // 5) % len(orgs)]
+        rays["ox"][j], rays["oy"][j], rays["oz"][j] = o
+    for j in np.nonzero(k % 7 == 2)[0]:
+        rays["tmax"][j] = tmaxs[(j // 7) % len(tmaxs)]
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes)
+        for any_hit in (False, True):
+            for packet in (1, 64):
+                print(f"non-finite rays any_hit={any_hit} packet={packet}", flush=True)
+                g = s.trace_host(rays, any_hit=any_hit, packet_size=packet)
+                o = oracle.trace(v, i, nodes, ids, rays, any_hit=any_hit, packet_size=packet)
+                assert np.array_equal(g["prim"], o["prim"]), (any_hit, packet)
+                assert _same_bits(g["t"], o["t"]), (any_hit, packet)
+                if any_hit:
+                    assert np.array_equal(g["occluded"], o["occluded"])
+                else:
+                    hit = o["prim"] >= 0
+                    assert _same_bits(g["u"][hit], o["u"][hit])
+                    assert _same_bits(g["v"][hit], o["v"][hit])
+        # the regular rays still hit
+        assert (o["occluded"][k % 3 == 1] == 1).any()
