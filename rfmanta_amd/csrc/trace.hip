@@ -59,23 +59,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 // ---------------------------------------------------------------------------
 // Packet traversal (one RayPacket per wave)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_min(float x) {
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float y = __shfl_xor(x, m, 64);
-    x = (x < y) ? x : y;
-  }
-  return x;
-}
-__device__ __forceinline__ float wave_max(float x) {
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float y = __shfl_xor(x, m, 64);
-    x = (x < y) ? y : x;
-  }
-  return x;
-}
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ int lowest(unsigned long long m) { return __builtin_ctzll(m); }
 __device__ __forceinline__ int highest(unsigned long long m) { return 63 - __builtin_clzll(m); }
+
+// The packet's IAData loop folds its rays in order with selects (DynBVH.cc:246-266):
+//   min: acc = (acc < x) ? acc : x from FLT_MAX;  max: acc = (acc < x) ? x : acc from -FLT_MAX.
+// The min fold takes a NaN (and restarts from the next ray after it) and keeps the later
+// of two equal values (+0/-0); the max fold skips NaNs and keeps the earlier. A butterfly
+// over (value, lane) pairs with those tie rules gives the fold's exact bits in any order.
+__device__ __forceinline__ float fold_min(float x, bool contrib, int lane) {
+  const unsigned long long nan = __ballot(contrib && x != x);
+  const int j = nan ? highest(nan) : -1;  // the last NaN: the fold restarts after it
+  float v = (contrib && lane > j) ? x : __builtin_inff();
+  int p = (contrib && lane > j) ? lane : -1;
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float y = __shfl_xor(v, m, 64);
+    const int q = __shfl_xor(p, m, 64);
+    const bool take = (y < v) || (!(v < y) && q > p);
+    v = take ? y : v;
+    p = take ? q : p;
+  }
+  if (j < 0) return (FLT_MAX < v) ? FLT_MAX : v;  // from the initial FLT_MAX
+  if (p < 0) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+  return v;
+}
+__device__ __forceinline__ float fold_max(float x, bool contrib, int lane) {
+  const bool use = contrib && x == x;
+  float v = use ? x : -__builtin_inff();
+  int p = use ? lane : 64;
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float y = __shfl_xor(v, m, 64);
+    const int q = __shfl_xor(p, m, 64);
+    const bool take = (v < y) || (!(y < v) && q < p);
+    v = take ? y : v;
+    p = take ? q : p;
+  }
+  return (-FLT_MAX < v) ? v : -FLT_MAX;  // from the initial -FLT_MAX
+}
 __device__ __forceinline__ unsigned long long above(int b) {
   return b >= 63 ? 0ull : (~0ull << (b + 1));
 }
@@ -141,10 +162,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_packet(DevScene sc, DevRays ra
   const bool contrib = st == 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    ia.min_rcp[a] = wave_min(contrib ? r.inv[a] : FLT_MAX);
-    ia.max_rcp[a] = wave_max(contrib ? r.inv[a] : -FLT_MAX);
-    ia.min_org[a] = wave_min(contrib ? r.o[a] : FLT_MAX);
-    ia.max_org[a] = wave_max(contrib ? r.o[a] : -FLT_MAX);
+    ia.min_rcp[a] = fold_min(r.inv[a], contrib, lane);
+    ia.max_rcp[a] = fold_max(r.inv[a], contrib, lane);
+    ia.min_org[a] = fold_min(r.o[a], contrib, lane);
+    ia.max_org[a] = fold_max(r.o[a], contrib, lane);
   }
 
   int pos = -1;
