@@ -255,6 +255,18 @@ int mh_trace(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, in
 int mh_trace_host(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
                   uint32_t flags, const mh_trace_opts* opts);
 
+/* No counterpart in the reference: mh_trace for callers that cannot allocate device
+ * memory (HIP graph capture, serving loops or harnesses that forbid hipMalloc). Same
+ * arguments and results as mh_trace; the single-ray kernel's traversal stack spill lives
+ * in a fixed device array of MH_NOALLOC_SCRATCH_BYTES per device instead of a buffer the
+ * scene allocates, and the call allocates nothing. The workgroup count is lowered until
+ * every lane's spill fits (results are unchanged); a tree too deep for one workgroup per
+ * compute unit to fit returns MH_E_UNSUPPORTED. All calls on one device share its array:
+ * calls that overlap on one device (other streams, other threads) race. */
+#define MH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mh_trace_noalloc(const mh_scene* scene, const mh_rays* rays, const mh_hits* hits, int64_t n,
+                     uint32_t flags, const mh_trace_opts* opts, void* stream);
+
 /* ---- Persistent multi-batch frames ------------------------------------------------
  * No reference counterpart: Manta traces one 64-ray packet per call. A frame is a fixed
  * table of ray batches (uneven sizes, each closest-hit or any-hit, device-resident SoA

@@ -164,6 +164,8 @@ _SIGS = {
                            C.POINTER(TraceOpts), _P]),
     "mh_trace_host": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,
                                 C.POINTER(TraceOpts)]),
+    "mh_trace_noalloc": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,
+                                   C.POINTER(TraceOpts), _P]),
     "mh_frame_create": (C.c_int, [_P, C.POINTER(Batch), C.c_int, C.POINTER(C.c_void_p)]),
     "mh_frame_set_batches": (C.c_int, [_P, C.POINTER(Batch), C.c_int, _P]),
     "mh_frame_set_config": (C.c_int, [_P, C.POINTER(LaunchConfig)]),

@@ -24,6 +24,17 @@ int ensure_init();
 int cu_count();
 const char* device_name();
 
+// The allocation-free trace's fixed scratch area (noalloc.hip). scratch_setup looks up
+// its device address on `device` (which must be current) once; ensure_init calls it. A
+// failed lookup is kept and reported by mh_trace_noalloc only.
+void scratch_setup(int device);
+// mh_trace with the single-ray kernel's stack spill in `spill` (spill_bytes long) instead
+// of a buffer the scene allocates: the workgroup count is lowered until every lane's
+// spill column fits; an error if fewer than one workgroup per compute unit would fit.
+int trace_fixed_spill(const mh_scene* s, const mh_rays* rays, const mh_hits* hits, int64_t n,
+                      uint32_t flags, const mh_trace_opts* opts, void* stream, void* spill,
+                      size_t spill_bytes);
+
 // Validates a batch launch configuration (frame_api.cpp); MH_OK or an error code.
 int check_launch_config(const mh_launch_config& c);
 

@@ -66,6 +66,7 @@ int ensure_init() {
   g_lib.cu_count = p.multiProcessorCount;
   g_lib.hbm = p.totalGlobalMem;
   std::snprintf(g_lib.name, sizeof(g_lib.name), "%s", p.gcnArchName);
+  scratch_setup(g_lib.device);
   g_lib.ready = true;
   return MH_OK;
 }
@@ -283,8 +284,11 @@ int choose_stack(const mh_trace_opts* o) {
   return (v == 8 || v == 16 || v == 32) ? v : 8;
 }
 
+// fixed_spill / fixed_bytes: a stack spill area the caller provides (mh_trace_noalloc);
+// null = the scene's per-stream spill buffer, grown here as needed.
 int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int64_t n,
-               uint32_t flags, const mh_trace_opts* opts, void* stream) {
+               uint32_t flags, const mh_trace_opts* opts, void* stream,
+               void* fixed_spill = nullptr, size_t fixed_bytes = 0) {
   mh_scene* s = const_cast<mh_scene*>(cs);
   if (!s || !rays || !hits) return fail(MH_E_INVALID, "null scene, rays or hits");
   if (!s->built) return fail(MH_E_STATE, "scene has no tree: call mh_scene_build first");
@@ -317,7 +321,22 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
     cfg.stack_lds = S;
     cfg.spill_stride = (int64_t)cfg.blocks * 256;
     const int64_t need = std::max<int64_t>(0, (int64_t)s->stack_bound(MH_LAYOUT_PAIR) - S);
-    if (need > 0) {
+    if (need > 0 && fixed_spill) {
+      // Fewer workgroups (the kernel strides over the rays, results are unchanged) until
+      // every lane's spill column fits the area; at least one workgroup per compute unit.
+      const int64_t per_block = need * 256 * 8;
+      const int64_t fit = (int64_t)(fixed_bytes / (size_t)per_block);
+      const int64_t floor_blocks = std::min<int64_t>(want, g_lib.cu_count);
+      if (fit < floor_blocks)
+        return fail(MH_E_UNSUPPORTED,
+                    "tree depth %u needs %lld bytes of stack spill for %lld workgroups; the "
+                    "fixed area holds %zu (use mh_trace, or a larger stack_lds)",
+                    s->depth, (long long)(per_block * floor_blocks), (long long)floor_blocks,
+                    fixed_bytes);
+      cfg.blocks = (int)std::min<int64_t>(cfg.blocks, fit);
+      cfg.spill_stride = (int64_t)cfg.blocks * 256;
+      cfg.spill = reinterpret_cast<int32_t*>(fixed_spill);
+    } else if (need > 0) {
       std::lock_guard<std::mutex> lk(s->spill_mu);
       DevBuf*& b = s->spill[stream];
       if (!b) b = new (std::nothrow) DevBuf();
@@ -334,6 +353,13 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
 }
 
 }  // namespace
+
+int mh_internal::trace_fixed_spill(const mh_scene* s, const mh_rays* rays, const mh_hits* hits,
+                                   int64_t n, uint32_t flags, const mh_trace_opts* opts,
+                                   void* stream, void* spill, size_t spill_bytes) {
+  if (!spill) return fail(MH_E_STATE, "no spill area");
+  return trace_impl(s, rays, hits, n, flags, opts, stream, spill, spill_bytes);
+}
 
 extern "C" {
 
