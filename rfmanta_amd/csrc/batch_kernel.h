@@ -46,6 +46,8 @@ constexpr int kKindAny = 2;
 constexpr int kModeLane = 0;        // per-lane refill from a wave pool
 constexpr int kModeWave = 1;        // whole-wave chunks from the shared counter
 constexpr int kModeWaveStatic = 2;  // whole-wave chunks on a static stride, no counter
+// Launch counters: [0] shared ray counter, [1] waves that left, [2..9] per-XCD ray counters.
+constexpr int kCounters = (int)(kBatchCounterBytes / sizeof(unsigned long long));
 
 // Last batch whose first ray number is <= g (empty batches are skipped over).
 __device__ __forceinline__ int find_batch(const int64_t* s_start, int nb, int64_t g) {
@@ -151,6 +153,91 @@ __device__ __forceinline__ bool step_pair(const DevScene& sc, Lane& L, LaneStack
     }
   }
   return false;
+}
+
+// Per-lane refill schedule, PAIR layout: one round of traverse.h's trace_pair loop, resumed
+// from the lane's state: the walk down inner nodes to a leaf (or to a miss), the leaf's
+// triangles and the pops to the next entry still worth visiting. The lanes of a wave walk
+// the inner nodes in lockstep (a lane that reached its leaf waits for the others), so the
+// wave issues each inner-node step once for all of them, as the plain kernel does; a
+// finished lane is refilled between rounds. Returns false when the ray is finished.
+template <bool ANY, int PRIM, int S, int BLOCK, bool FAST>
+__device__ __forceinline__ bool round_pair(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
+  unsigned long long ntri = 0;
+  const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
+  bool missed = false;
+  while (L.cur >= 0) {
+    const float4 A = fetch_record(pt, L.cur, 0), B = fetch_record(pt, L.cur, 16),
+                 C = fetch_record(pt, L.cur, 32), Mf = fetch_record(pt, L.cur, 48);
+    const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
+                             __float_as_int(Mf.w));
+    float tn0, tn1;
+    const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
+    const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
+    const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : __builtin_amdgcn_ubfe(L.r.dpos, M.z, 1) == 0;
+    if (h0 && h1) st.push(one_first ? M.x : M.y, one_first ? tn0 : tn1);
+    if (!(h0 || h1)) {
+      missed = true;
+      break;
+    }
+    L.cur = (h1 && (one_first || !h0)) ? M.y : M.x;
+  }
+  if (!missed) {
+    const int code = ~L.cur;
+    const int first = code >> kLeafCountBits;
+    int cnt = code & kLeafCountInline;
+    if (cnt == kLeafCountInline) cnt = sc.big_count[first];
+    if (leaf<ANY, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)) return false;
+  }
+  while (!st.empty()) {
+    int ref;
+    float t;
+    st.pop(ref, t);
+    if (t <= L.h.minT) {
+      L.cur = ref;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Next chunk [base, end) of ray numbers for a wave of the per-lane refill schedule, or
+// base = -1 once there are none. One range: a shared counter over all rays (ctr[0]). Per
+// XCD: the rays are cut into eight contiguous ranges with a counter each (ctr[2 + k]); a
+// wave starts on the range of its XCD (workgroups are dealt to the 8 XCDs round-robin), so
+// each XCD's L2 serves the nodes of one region of the rays, and moves on to the next
+// ranges in turn when its own is used up. xr (wave-uniform) counts the ranges this wave
+// found used up.
+__device__ __forceinline__ void grab_chunk(unsigned long long* ctr, int64_t C, int64_t total,
+                                           bool per_xcd, int& xr, int lane, int64_t& base,
+                                           int64_t& end) {
+  long long b = -1, e = -1;
+  int nx = xr;
+  if (lane == 0) {
+    if (!per_xcd) {
+      const int64_t o = (int64_t)atomicAdd(ctr, (unsigned long long)C);
+      if (o < total) {
+        b = o;
+        e = o + C < total ? o + C : total;
+      }
+    } else {
+      const int x0 = (int)(blockIdx.x & 7);
+      while (nx < 8) {
+        const int k = (x0 + nx) & 7;
+        const int64_t lo = total * k / 8, hi = total * (k + 1) / 8;
+        const int64_t o = (int64_t)atomicAdd(ctr + 2 + k, (unsigned long long)C);
+        if (lo + o < hi) {
+          b = lo + o;
+          e = b + C < hi ? b + C : hi;
+          break;
+        }
+        ++nx;
+      }
+    }
+  }
+  base = (int64_t)__shfl(b, 0, 64);
+  end = (int64_t)__shfl(e, 0, 64);
+  xr = __shfl(nx, 0, 64);
 }
 
 // Starts ray li of batch B in lane L; returns false if it finished immediately.
@@ -286,6 +373,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
     bool pending = false;
     bool exhausted = false;
     int64_t pool_next = 0, pool_end = 0;  // wave-uniform
+    int xr = 0;                           // ray ranges this wave found empty (per-XCD grabs)
 
     for (;;) {
       const unsigned long long idle = __ballot(!active);
@@ -300,14 +388,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
         int64_t g = -1;
         if (!active && rank < avail) g = pool_next + rank;
         if (n_idle > avail) {
-          unsigned long long b = 0;
-          if (lane == 0) b = atomicAdd(ctr, (unsigned long long)C);
-          const int64_t base = (int64_t)__shfl((long long)b, 0, 64);
-          if (base >= total) {
+          int64_t base, end;
+          grab_chunk(ctr, C, total, xcd_remap != 0, xr, lane, base, end);
+          if (base < 0) {
             exhausted = true;
             pool_next = pool_end = total;
           } else {
-            const int64_t end = base + C < total ? base + C : total;
             if (!active && rank >= avail && base + (rank - avail) < end) g = base + (rank - avail);
             const int64_t taken = n_idle - avail;
             pool_next = base + taken < end ? base + taken : end;
@@ -328,17 +414,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
         if (active) {
           bool more;
           if (KIND == kKindClosest)
-            more = fast ? step_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
-                        : step_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
+                        : round_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
           else if (KIND == kKindAny)
-            more = fast ? step_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
-                        : step_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
+                        : round_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
           else if (L.any)
-            more = fast ? step_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
-                        : step_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
+                        : round_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
           else
-            more = fast ? step_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
-                        : step_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
+                        : round_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
           if (!more) {
             active = false;
             pending = true;
@@ -356,8 +442,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
     // The last wave out clears the counters for the next launch.
     const unsigned long long old = atomicAdd(ctr + 1, 1ull);
     if (old == (unsigned long long)n_waves - 1) {
-      atomicExch(ctr + 0, 0ull);
-      atomicExch(ctr + 1, 0ull);
+      for (int k = 0; k < kCounters; ++k) atomicExch(ctr + k, 0ull);
     }
   }
 }

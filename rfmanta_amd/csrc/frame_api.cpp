@@ -12,6 +12,7 @@
 #include <new>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "internal.h"
@@ -276,13 +277,13 @@ int mh_frame_create(const mh_scene* scene, const mh_batch* batches, int n_batche
   f->total = total;
   f->n_any = n_any;
   f->cfg = mh_launch_config{-1, 0, 0, 0, 0};
-  if ((rc = f->table.reserve(host.size())) || (rc = f->counters.reserve(16)) ||
+  if ((rc = f->table.reserve(host.size())) || (rc = f->counters.reserve(mh::kBatchCounterBytes)) ||
       (rc = prepare(f, kDefault))) {
     delete f;
     return rc;
   }
   hipError_t e = hipMemcpy(f->table.p, host.data(), host.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(f->counters.p, 0, 16);
+  if (e == hipSuccess) e = hipMemset(f->counters.p, 0, mh::kBatchCounterBytes);
   if (e != hipSuccess) {
     delete f;
     return fail(MH_E_HIP, "frame upload: %s", hipGetErrorString(e));
@@ -434,15 +435,21 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
       t.stack_lds = s;
       if ((rc = trial(t))) return rc;
     }
-  // Stage 4: per-lane refill thresholds.
+  // Stage 4: per-lane refill, with shapes of its own: its waves stay resident and take
+  // rays in chunks, so grids of about the resident wave count (six or seven per SIMD)
+  // and chunks that leave every wave several grabs; the best ring and layout so far.
   base = best;
-  for (int r : {8, 16, 32})
-    for (int c : {64, 256}) {
-      mh_launch_config t = base;
-      t.refill = r;
-      t.chunk = c;
-      if ((rc = trial(t))) return rc;
-    }
+  for (int r : {16, 32, 48})
+    for (int c : {64, 128, 256})
+      for (const auto& bp : {std::pair<int, int>{256, 6}, {256, 7}, {128, 12}, {128, 14}}) {
+        mh_launch_config t = base;
+        t.refill = r;
+        t.chunk = c;
+        t.block = bp.first;
+        t.blocks_per_cu = bp.second;
+        t.xcd_remap = 0;
+        if ((rc = trial(t))) return rc;
+      }
   if ((rc = prepare(f, best))) return rc;
   if (cache_path) {
     char buf[256];

@@ -101,10 +101,13 @@ struct BatchTable {
   BatchDev batches[1];  // nb entries
 };
 
+// Bytes of a batch launch's counters (ray counters, exit count; zero between launches).
+constexpr size_t kBatchCounterBytes = 10 * sizeof(unsigned long long);
+
 struct BatchLaunch {
   const BatchTable* table;
   int nb;  // host copy, for argument checks only
-  unsigned long long* counters;  // device u64[2], zero between launches (self-resetting)
+  unsigned long long* counters;  // device u64[10], zero between launches (self-resetting)
   int32_t* spill;
   int64_t spill_stride;  // blocks * block
   int blocks;

@@ -512,8 +512,8 @@ int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_serve
   for (int k = 0; k < n_scenes; ++k) {
     auto& p = *s->per[k];
     HIP_OK(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
-    if ((rc = p.table.reserve(table_bytes())) || (rc = p.counters.reserve(16))) return rc;
-    HIP_OK(hipMemset(p.counters.p, 0, 16));
+    if ((rc = p.table.reserve(table_bytes())) || (rc = p.counters.reserve(mh::kBatchCounterBytes))) return rc;
+    HIP_OK(hipMemset(p.counters.p, 0, mh::kBatchCounterBytes));
     HIP_OK(hipHostMalloc(&p.host_table, table_bytes(), hipHostMallocDefault));
     p.blocks = s->launch.blocks_per_cu * mh_internal::cu_count();
     p.spill_stride = (int64_t)p.blocks * s->launch.block;
