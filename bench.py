@@ -6,7 +6,8 @@ Sections (all on synthetic inputs, generated at start-up and resident in HBM whe
       pinhole primary rays (coherent, 8x8 tiles), one cosine-weighted diffuse bounce ray
       per primary hit (incoherent, PCG32 seed 1) and one shadow ray per hit (any-hit).
       One step = one frame of closest-hit work, primary + bounce rays, traced by the
-      batched form: one persistent launch over both batches (autotuned configuration).
+      batched form: one persistent launch over both batches (pinned launch configuration,
+      the autotuner's pick reported beside it).
       `value` is that frame's closest-hit rays per second; the plain path (one launch per
       ray set) is timed beside it.
   C2  64 batches of 1k-2M rays (log-uniform; 40% primary, 30% shadow, 30% bounce; seed 1)
@@ -60,6 +61,11 @@ TRI_OPS = 45
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 PROFILE_JSON = "profiles/counters_trace_batch_heightfield1M.json"
+# Launch configuration of the timed C1 frame (the autotuner's usual pick on MI355X: static
+# stride, 24 workgroups of 256 lanes per CU, LDS ring of 8); pinned so that the counter
+# profile of the kernel is taken on the very launch that is timed.
+C1_CONFIG = {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 24, "refill": 64,
+             "chunk": 0, "xcd_remap": 0}
 KERNEL_SOURCES = ("batch_kernel.h", "traverse.h", "trace_common.h", "mh_device.h",
                   "batch_ks_pair.hip")
 
@@ -88,9 +94,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--skip", default="",
-                    help="comma list of sections to skip: singles (per-set frames), c2, c3, c4")
-    ap.add_argument("--c1-config", default="",
-                    help="JSON launch configuration for the C1 frame instead of autotuning")
+                    help="comma list of sections to skip: singles (per-set frames), dropin, "
+                         "build, c2, c3, c4")
+    ap.add_argument("--c1-config", default=json.dumps(C1_CONFIG),
+                    help="JSON launch configuration of the timed C1 frame ('' = the "
+                         "autotuner's pick)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner run reported beside the C1 frame")
     ap.add_argument("--c4-frames", type=int, default=100)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration)")
@@ -190,6 +200,13 @@ def main() -> int:
 
     # ---------------------------------------------------------------- batched C1 frame
     frame = api.Frame(scene, [(d_prim, h_prim, False), (d_b, h_b, False)])
+    autotune = None
+    if args.c1_config and not args.no_tune:
+        # the autotuner's pick on this box, reported beside the pinned configuration
+        t0 = time.time()
+        pick, pick_ms = frame.autotune(iters=3, cache_path="/tmp/mh_tune.json", force=True)
+        autotune = {"config": pick, "ms": round(pick_ms, 4), "s": round(time.time() - t0, 1)}
+        log(f"C1 frame autotune {autotune}")
     if args.c1_config:
         frame.set_config(json.loads(args.c1_config))
         tuned = frame.config()
@@ -295,7 +312,7 @@ def main() -> int:
         "config": {"workload": "heightfield 1000x500 (997,002 tris), 1920x1080 primary + "
                                "1 diffuse bounce per hit, closest-hit, one batched launch",
                    "primary_rays": n_prim, "bounce_rays": n_b, "shadow_rays": n_s,
-                   "launch": tuned, "parallelism": "1 GPU"},
+                   "launch": tuned, "autotune": autotune, "parallelism": "1 GPU"},
         "coherent_mrays": single["coherent"]["mrays"],
         "incoherent_mrays": single["incoherent"]["mrays"],
         "shadow_anyhit_mrays": single["shadow_anyhit"]["mrays"],
@@ -307,6 +324,11 @@ def main() -> int:
         "roofline": roofline,
     }
     frame.close()
+
+    if "dropin" not in skip:
+        out["drop_in"] = bench_dropin(api, L, scene, verts, idx, prim_rays,
+                                      {"primary": (d_prim, h_prim, n_prim),
+                                       "bounce": (d_b, h_b, n_b)}, args)
 
     if "build" not in skip:
         out["build"] = bench_build(api, L, verts, idx, build_s,
@@ -336,6 +358,57 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     scene.close()
     return 0
+
+
+def bench_dropin(api, L, scene, verts, idx, prim_rays, sets, args) -> dict:
+    """The reference-exact packet path and the Manta drop-in on the C1 rays.
+
+    packet_kernel: k_trace_packet (64-ray packets on one wave each, DynBVH::intersectNode
+    semantics) on the device-resident primary rays in 8x8-tile order and on the bounce rays
+    in their order, timed with HIP events. hipdynbvh: HipDynBVH::intersect called by one
+    render thread per host core with 64-ray RayPackets (the way Manta's workers call
+    DynBVH), over the primary rays; the shim gathers concurrent packets into launches and
+    copies rays and hits over PCIe; wall time of the intersect phase (the build in
+    preprocess is not timed)."""
+    res = {"packet_kernel": {}}
+    opts = api.trace_opts(64)
+    for name, (dr, dh, n) in sets.items():
+        ms = C.c_float()
+        r, h = dr.struct(), dh.struct()
+        L.check(L.load().mh_time_trace(scene.h, C.byref(r), C.byref(h), dr.n, 0, C.byref(opts),
+                                       None, args.steps, C.byref(ms)))
+        res["packet_kernel"][f"{name}_mrays"] = round(mrays(n, ms.value / args.steps), 1)
+    from rfmanta_amd import build as B
+    lib = C.CDLL(str(B.HARNESS))
+    lib.mh_manta_render.restype = C.c_int
+
+    class Io(C.Structure):
+        _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax",
+                                              "masked", "prim", "t", "u", "v", "occluded")]
+    n = len(prim_rays["ox"])
+    arr = {k: np.ascontiguousarray(prim_rays[k], np.float32)
+           for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")}
+    outp = {"prim": np.empty(n, np.int32), "t": np.empty(n, np.float32),
+            "u": np.empty(n, np.float32), "v": np.empty(n, np.float32)}
+    io = Io(*[arr[k].ctypes.data for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")], None,
+            *[outp[k].ctypes.data for k in ("prim", "t", "u", "v")], None)
+    v = np.ascontiguousarray(verts, np.float32)
+    ii = np.ascontiguousarray(idx, np.uint32)
+    err = C.create_string_buffer(512)
+    threads = host_cores()
+    sec = C.c_double()
+    rc = lib.mh_manta_render(C.c_void_p(v.ctypes.data), C.c_uint32(len(v)),
+                             C.c_void_p(ii.ctypes.data), C.c_uint32(len(ii)), 0, threads,
+                             C.byref(io), C.c_int64(n), 0, 0, err, 512, C.byref(sec))
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    res["hipdynbvh"] = {"threads": threads, "rays": n, "intersect_s": round(sec.value, 4),
+                        "mrays": round(n / sec.value / 1e6, 1)}
+    # the shim's packets agree with the packet kernel on the device (same traversal)
+    dp = sets["primary"][1].download()
+    res["hipdynbvh"]["same_as_packet_kernel"] = bool(np.array_equal(outp["prim"], dp["prim"]))
+    log(f"drop-in: {res}")
+    return res
 
 
 def bench_build(api, L, verts, idx, host_build_s, sets, args) -> dict:

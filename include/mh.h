@@ -33,6 +33,7 @@ extern "C" {
 #define MH_E_STATE (-4)       /* call out of order (e.g. trace before build) */
 #define MH_E_UNSUPPORTED (-5) /* input outside a documented limit */
 #define MH_E_IO (-6)          /* file could not be read or written */
+#define MH_E_CANCELED (-7)    /* request cancelled (mh_server_cancel) */
 
 /* Trace flags. */
 #define MH_CLOSEST_HIT 0u
@@ -363,10 +364,11 @@ int mh_gen_bounce_rays(const mh_scene* scene, const mh_rays* primary, const mh_h
  * camera over one of its scenes: closest-hit primary rays and, optionally, one shadow ray
  * and one diffuse bounce ray per primary ray) with continuous batching: each round of its
  * scheduler thread admits waiting requests in arrival order while their device buffers
- * fit the HBM budget, then launches one persistent batch trace per scene over a chunk of
- * every admitted request's remaining rays (scenes on separate streams, overlapping);
- * requests whose rays are all traced have their results copied to pinned host staging
- * and their device buffers freed. Per-ray results equal mh_gen_camera_rays +
+ * fit the HBM budget, then queues one persistent batch trace per scene over a chunk of
+ * every admitted request's remaining rays (scenes on separate streams, overlapping; the
+ * next table of a scene is built while its launch runs, without waiting for it);
+ * requests whose rays are all queued have their results copied to pinned host staging on
+ * a copy stream that overlaps later launches, and their device buffers freed. Per-ray results equal mh_gen_camera_rays +
  * mh_trace (packet_size 1) + mh_gen_shadow_rays / mh_gen_bounce_rays + mh_trace. */
 typedef struct mh_server mh_server;
 
@@ -409,7 +411,7 @@ typedef struct mh_request_info {
 typedef struct mh_server_stats {
   uint64_t submitted, completed, rejected;
   int64_t rays;        /* rays traced since creation */
-  double busy_ms;      /* wall time of scheduler rounds */
+  double busy_ms;      /* wall time with requests in flight */
   int active, pending;
   size_t hbm_in_use;
   size_t host_in_use;
@@ -423,6 +425,10 @@ int mh_server_submit(mh_server* server, const mh_request* req, uint64_t* id);
 /* Waits for a request and copies its results out (out may be NULL); each id once. */
 int mh_server_wait(mh_server* server, uint64_t id, const mh_request_result* out,
                    mh_request_info* info);
+/* Cancels a request: a queued one at once, one in flight once the device work already
+ * queued for it has passed (no further chunks are handed out). mh_server_wait then reports
+ * MH_E_CANCELED. MH_E_STATE if it has already finished. */
+int mh_server_cancel(mh_server* server, uint64_t id);
 int mh_server_get_stats(mh_server* server, mh_server_stats* stats);
 double mh_server_clock_ms(const mh_server* server);
 /* Finishes the requests already submitted, then stops; results not yet collected are

@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmh.so"
 
 MH_OK = 0
 MH_E_INVALID, MH_E_HIP, MH_E_NOMEM, MH_E_STATE, MH_E_UNSUPPORTED, MH_E_IO = -1, -2, -3, -4, -5, -6
+MH_E_CANCELED = -7
 MH_CLOSEST_HIT = 0
 MH_ANY_HIT = 1
 MH_PRIM_KENSLER_SHIRLEY = 0
@@ -186,6 +187,7 @@ _SIGS = {
                                    C.POINTER(C.c_void_p)]),
     "mh_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     "mh_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(RequestResult), C.POINTER(RequestInfo)]),
+    "mh_server_cancel": (C.c_int, [_P, C.c_uint64]),
     "mh_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "mh_server_clock_ms": (C.c_double, [_P]),
     "mh_server_destroy": (C.c_int, [_P]),

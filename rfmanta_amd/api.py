@@ -450,6 +450,10 @@ class Server:
                    admit_ms=info.admit_ms, done_ms=info.done_ms)
         return out
 
+    def cancel(self, rid: int) -> None:
+        """Cancels request `rid` (queued: at once; in flight: behind its queued work)."""
+        L.check(L.load().mh_server_cancel(self.h, rid))
+
     def stats(self) -> dict:
         st = L.ServerStats()
         L.check(L.load().mh_server_get_stats(self.h, C.byref(st)))

@@ -441,7 +441,7 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
   base = best;
   for (int r : {16, 32, 48})
     for (int c : {64, 128, 256})
-      for (const auto& bp : {std::pair<int, int>{256, 6}, {256, 7}, {128, 12}, {128, 14}}) {
+      for (const auto& bp : {std::pair<int, int>{256, 5}, {256, 6}, {256, 7}, {128, 12}, {128, 14}}) {
         mh_launch_config t = base;
         t.refill = r;
         t.chunk = c;

@@ -1,21 +1,25 @@
 // Serving loop (mh_server_*, include/mh.h): resident scenes, HBM-budget admission and
 // continuous batching over the persistent batch kernel.
 //
-// Threads: callers submit and wait; one scheduler thread owns the device work. Each
-// scheduler round
+// Threads: callers submit, wait and cancel; one scheduler thread owns the device work and
+// never waits for a launch it could overlap. Each step of the scheduler
 //   1. admits queued requests in arrival order while their device buffers fit the device
-//      arena (the HBM budget) and fewer than max_active are in flight, generating their
-//      camera rays on the device;
-//   2. builds, per scene, a batch table with one chunk of each remaining ray set of every
-//      admitted request (primary closest-hit first; then its shadow (any-hit) and bounce
-//      (closest-hit) rays, generated on the device from the primary hits) and launches
-//      one persistent trace per scene, each scene on its own stream so that the launches
-//      overlap;
-//   3. waits for the launches; a request whose ray sets are all traced gets a range of the
-//      pinned host arena, its results are copied there and its device range is freed.
-// mh_server_wait copies a finished request's results out of the host arena and frees
-// that range. A request joins the device work at the first round after it fits, so large
-// requests never hold small ones back for more than one round.
+//      arena (the HBM budget) and fewer than max_active are in flight, and queues their
+//      camera rays on their scene's stream;
+//   2. for every scene with fewer than two launches queued, fills the free one of the
+//      scene's two batch tables with one chunk of each remaining ray set of its admitted
+//      requests (primary closest-hit first; once every primary chunk of a request is
+//      queued, its shadow (any-hit) and bounce (closest-hit) rays are generated on the
+//      device by a kernel queued behind them, and their chunks join later tables) and
+//      queues the table upload and one persistent trace; scenes have their own streams, so
+//      their launches overlap, and the next table is built while a launch runs;
+//   3. for a request whose rays are all queued, reserves a range of the pinned host arena
+//      and queues the result copies on a copy stream behind an event of its last launch,
+//      so copies overlap the next launches;
+//   4. retires launches and copies whose events have completed; a request is finished, and
+//      its device range freed, when its copies are done.
+// mh_server_wait copies a finished request's results out of the host arena and frees that
+// range. A request joins the device work with the next table of its scene after it fits.
 
 #include <hip/hip_runtime_api.h>
 
@@ -122,6 +126,10 @@ struct Request {
   RaySet prim;
   std::vector<RaySet> sec;  // shadow and/or bounce
   bool primary_done = false, sec_generated = false, done = false;
+  bool handed_all = false;       // every chunk of every ray set is in a queued launch
+  bool cancelled = false;        // mh_server_cancel: no further chunks are handed out
+  hipEvent_t retire = nullptr;   // completes when the request's copies (or work) are done
+  hipEvent_t after = nullptr;    // its last launch, which the copies wait for
   // results in the host arena
   int32_t* h_prim = nullptr;
   float* h_t = nullptr;
@@ -153,15 +161,20 @@ struct mh_server {
   std::unique_ptr<Arena> arena;
   char* host_mem = nullptr;
   std::unique_ptr<Arena> host_arena;
-  // per scene: stream, batch table (device + pinned host), counters, spill
+  // per scene: stream, two batch tables (device + pinned host) used in turn, counters,
+  // spill, and the launches queued on the stream (with the table each one reads)
   struct PerScene {
     hipStream_t stream = nullptr;
-    DevBuf table, counters, spill;
-    void* host_table = nullptr;
+    DevBuf table[2], counters, spill;
+    void* host_table[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};  // completion of the launch reading table k
+    std::deque<int> queued;                   // tables of launches not yet seen complete
     int64_t spill_stride = 0;
     int blocks = 0;
   };
   std::vector<std::unique_ptr<PerScene>> per;
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> free_events;
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -180,14 +193,23 @@ struct mh_server {
   ~mh_server() {
     for (auto& p : per) {
       if (p->stream) (void)hipStreamDestroy(p->stream);
-      if (p->host_table) (void)hipHostFree(p->host_table);
+      for (int k = 0; k < 2; ++k) {
+        if (p->host_table[k]) (void)hipHostFree(p->host_table[k]);
+        if (p->done[k]) (void)hipEventDestroy(p->done[k]);
+      }
     }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    for (hipEvent_t e : free_events) (void)hipEventDestroy(e);
     if (host_mem) (void)hipHostFree(host_mem);
   }
-  bool progressed = false;  // the last round launched or finished something
+  bool progressed = false;  // the last step queued or retired something
+  double busy_since = -1;   // server clock when requests were last seen in flight
   void run();
-  int round();
+  int step();
+  int queue_scene(int k, const std::vector<std::shared_ptr<Request>>& work, int64_t& traced);
+  int event(hipEvent_t& e);
   void admit_locked(std::vector<std::shared_ptr<Request>>& fresh);
+  bool gpu_busy() const;
 };
 
 namespace {
@@ -269,91 +291,93 @@ void mh_server::admit_locked(std::vector<std::shared_ptr<Request>>& fresh) {
   }
 }
 
-int mh_server::round() {
-  std::vector<std::shared_ptr<Request>> fresh, work;
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    admit_locked(fresh);
-    work = active;
+// An event from the pool (created on first use; timing disabled: only ordering).
+int mh_server::event(hipEvent_t& e) {
+  if (!free_events.empty()) {
+    e = free_events.back();
+    free_events.pop_back();
+    return MH_OK;
   }
-  // Camera rays of newly admitted requests, on their scene's stream.
-  for (auto& r : fresh) {
-    mh::CameraDev cd;
-    int rc = mh_internal::camera_setup(r->cam, cd);
-    if (rc) {
-      r->status = rc;
-      continue;
-    }
-    HIP_OK((hipError_t)mh::launch_camera(cd, ray_out(r->prim.rays), per[r->scene]->stream));
-  }
-  // One batch table per scene: a chunk of each remaining ray set of each request.
-  const int ns = (int)scenes.size();
-  std::vector<int> nb(ns, 0), n_any(ns, 0);
-  std::vector<int64_t> total(ns, 0);
+  HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return MH_OK;
+}
+
+bool mh_server::gpu_busy() const {
+  for (auto& p : per)
+    if (!p->queued.empty()) return true;
+  for (auto& r : active)
+    if (r->retire) return true;
+  return false;
+}
+
+// Fills scene k's free table with a chunk of each remaining ray set of its requests and
+// queues the upload and the launch, then the secondary-ray generation of requests whose
+// primary rays are now all queued.
+int mh_server::queue_scene(int k, const std::vector<std::shared_ptr<Request>>& work,
+                           int64_t& traced) {
+  PerScene& P = *per[k];
+  const int slot = P.queued.empty() ? 0 : 1 - P.queued.back();
+  mh::BatchTable* t = reinterpret_cast<mh::BatchTable*>(P.host_table[slot]);
+  int nb = 0, n_any = 0;
+  int64_t total = 0;
   auto add = [&](Request& r, RaySet& s) {
-    const int k = r.scene;
-    if (s.next >= r.n || nb[k] >= kTableCap) return;
+    if (s.next >= r.n || nb >= kTableCap) return;
     const int64_t c = std::min<int64_t>(opts.chunk_rays, r.n - s.next);
-    mh::BatchTable* t = reinterpret_cast<mh::BatchTable*>(per[k]->host_table);
     mh::BatchDev d = chunk(s, c);
-    d.start = total[k];
-    t->batches[nb[k]++] = d;
-    n_any[k] += s.any ? 1 : 0;
-    total[k] += c;
+    d.start = total;
+    t->batches[nb++] = d;
+    n_any += s.any ? 1 : 0;
+    total += c;
     s.next += c;
   };
   for (auto& r : work) {
-    if (r->status != MH_OK || r->done) continue;
-    if (!r->primary_done)
+    if (r->scene != k || r->status != MH_OK || r->cancelled || r->handed_all) continue;
+    if (r->prim.next < r->n)
       add(*r, r->prim);
     else if (r->sec_generated)
-      for (auto& s : r->sec) add(*r, s);
+      for (auto& x : r->sec) add(*r, x);
   }
-  for (int k = 0; k < ns; ++k) {
-    if (nb[k] == 0) continue;
-    mh::BatchTable* t = reinterpret_cast<mh::BatchTable*>(per[k]->host_table);
-    t->nb = nb[k];
-    t->total = total[k];
-    const size_t bytes = offsetof(mh::BatchTable, batches) + sizeof(mh::BatchDev) * nb[k];
-    HIP_OK(hipMemcpyAsync(per[k]->table.p, t, bytes, hipMemcpyHostToDevice, per[k]->stream));
-    // A rotating update between rounds can deepen the tree: grow the spill area first.
+  if (nb > 0) {
+    t->nb = nb;
+    t->total = total;
+    const size_t bytes = offsetof(mh::BatchTable, batches) + sizeof(mh::BatchDev) * nb;
+    HIP_OK(hipMemcpyAsync(P.table[slot].p, t, bytes, hipMemcpyHostToDevice, P.stream));
+    // A rotating update between requests can deepen the tree: grow the spill area first
+    // (the stream's earlier launches may still use the old one).
     const int64_t need = std::max<int64_t>(
         0, (int64_t)scenes[k]->stack_bound(launch.layout) - launch.stack_lds);
-    const size_t spill_bytes = (size_t)need * per[k]->spill_stride * 8;
-    if (spill_bytes > per[k]->spill.bytes) {
-      HIP_OK(hipStreamSynchronize(per[k]->stream));
-      if (int rc = per[k]->spill.reserve(spill_bytes)) return rc;
+    const size_t spill_bytes = (size_t)need * P.spill_stride * 8;
+    if (spill_bytes > P.spill.bytes) {
+      HIP_OK(hipStreamSynchronize(P.stream));
+      if (int rc = P.spill.reserve(spill_bytes)) return rc;
     }
     mh::BatchLaunch bl{};
-    bl.table = per[k]->table.as<const mh::BatchTable>();
-    bl.nb = nb[k];
-    bl.counters = per[k]->counters.as<unsigned long long>();
-    bl.spill = per[k]->spill.as<int32_t>();
-    bl.spill_stride = per[k]->spill_stride;
-    bl.blocks = per[k]->blocks;
+    bl.table = P.table[slot].as<const mh::BatchTable>();
+    bl.nb = nb;
+    bl.counters = P.counters.as<unsigned long long>();
+    bl.spill = P.spill.as<int32_t>();
+    bl.spill_stride = P.spill_stride;
+    bl.blocks = P.blocks;
     bl.block = launch.block;
     bl.stack_lds = launch.stack_lds;
     bl.layout = launch.layout;
     bl.refill = launch.refill;
     bl.chunk = launch.chunk;
     bl.xcd_remap = launch.xcd_remap;
-    bl.kinds = n_any[k] == 0 ? 1 : (n_any[k] == nb[k] ? 2 : 0);
-    HIP_OK((hipError_t)mh::launch_trace_batch(scenes[k]->dev(), bl, per[k]->stream));
+    bl.kinds = n_any == 0 ? 1 : (n_any == nb ? 2 : 0);
+    HIP_OK((hipError_t)mh::launch_trace_batch(scenes[k]->dev(), bl, P.stream));
+    HIP_OK(hipEventRecord(P.done[slot], P.stream));
+    P.queued.push_back(slot);
+    traced += total;
   }
-  for (int k = 0; k < ns; ++k) HIP_OK(hipStreamSynchronize(per[k]->stream));
-  int64_t traced = 0;
-  for (int k = 0; k < ns; ++k) traced += total[k];
-  // Progress: primary done -> secondary generation; all done -> results to the host.
-  std::vector<std::shared_ptr<Request>> completed;
+  // Secondary rays of requests whose primary chunks are all queued: generated on the
+  // stream behind those launches, traced by later tables.
   for (auto& r : work) {
-    if (r->done) continue;
-    if (r->status != MH_OK) {
-      completed.push_back(r);
-      continue;
-    }
-    if (!r->primary_done && r->prim.next >= r->n) {
-      r->primary_done = true;
-      const mh_scene* sc = scenes[r->scene];
+    if (r->scene != k || r->status != MH_OK || r->cancelled || r->sec_generated) continue;
+    if (r->prim.next < r->n) continue;
+    r->primary_done = true;
+    if (!r->sec.empty()) {
+      const mh_scene* sc = scenes[k];
       mh::SecondaryArgs a{};
       const mh_rays& p = r->prim.rays;
       a.in = mh::DevRays{p.ox, p.oy, p.oz, p.dx, p.dy, p.dz, p.tmax, nullptr};
@@ -364,70 +388,150 @@ int mh_server::round() {
       for (int i = 0; i < 3; ++i) a.light[i] = opts.light[i];
       a.seed = r->cam.seed + 1;
       a.n = r->n;
-      for (auto& s : r->sec) {
-        a.out = ray_out(s.rays);
-        HIP_OK((hipError_t)mh::launch_secondary(a, s.any ? 0 : 1, per[r->scene]->stream));
+      for (auto& x : r->sec) {
+        a.out = ray_out(x.rays);
+        HIP_OK((hipError_t)mh::launch_secondary(a, x.any ? 0 : 1, P.stream));
       }
-      r->sec_generated = true;
     }
-    bool all = r->primary_done;
-    for (auto& s : r->sec) all = all && s.next >= r->n;
-    if (all) completed.push_back(r);
+    r->sec_generated = true;
   }
-  // Host ranges for the finished requests (those that do not fit wait for collection).
-  std::vector<std::shared_ptr<Request>> copied;
+  for (auto& r : work) {
+    if (r->scene != k || r->status != MH_OK || r->cancelled || r->handed_all) continue;
+    bool all = r->sec_generated;
+    for (auto& x : r->sec) all = all && x.next >= r->n;
+    if (all) r->handed_all = true;
+  }
+  return MH_OK;
+}
+
+int mh_server::step() {
+  std::vector<std::shared_ptr<Request>> fresh, work;
   {
     std::lock_guard<std::mutex> lk(mu);
-    for (auto& r : completed) {
-      if (r->status != MH_OK) {
-        copied.push_back(r);
+    admit_locked(fresh);
+    work = active;
+  }
+  bool moved = !fresh.empty();
+  // Camera rays of newly admitted requests, on their scene's stream.
+  for (auto& r : fresh) {
+    mh::CameraDev cd;
+    int rc = mh_internal::camera_setup(r->cam, cd);
+    if (rc) {
+      r->status = rc;
+      continue;
+    }
+    HIP_OK((hipError_t)mh::launch_camera(cd, ray_out(r->prim.rays), per[r->scene]->stream));
+  }
+  // Launches seen complete free their tables.
+  for (auto& P : per) {
+    while (!P->queued.empty()) {
+      const hipError_t q = hipEventQuery(P->done[P->queued.front()]);
+      if (q == hipErrorNotReady) break;
+      HIP_OK(q);
+      P->queued.pop_front();
+      moved = true;
+    }
+  }
+  // Next launch of every scene with a free table.
+  int64_t traced = 0;
+  for (int k = 0; k < (int)scenes.size(); ++k)
+    if (per[k]->queued.size() < 2)
+      if (int rc = queue_scene(k, work, traced)) return rc;
+  moved = moved || traced > 0;
+  // Requests with all their rays queued: host ranges and result copies behind their
+  // scene's launches; cancelled requests: an event behind the work already queued.
+  std::vector<std::shared_ptr<Request>> to_copy;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    st.rays += traced;
+    for (auto& r : work) {
+      if (r->retire || r->done) continue;
+      if (r->status != MH_OK || r->cancelled) {
+        to_copy.push_back(r);
         continue;
       }
+      if (!r->handed_all) continue;
       const size_t hb = host_bytes(r->n, opts.shadows != 0, opts.bounces != 0);
       size_t off;
-      if (!host_arena->alloc(hb, off)) continue;
+      if (!host_arena->alloc(hb, off)) continue;  // waits for collections
       r->host_off = off;
       r->host_bytes = hb;
       r->has_host = true;
-      copied.push_back(r);
+      to_copy.push_back(r);
     }
   }
-  for (auto& r : copied) {
-    if (r->status != MH_OK) continue;
+  for (auto& r : to_copy) {
     hipStream_t s = per[r->scene]->stream;
+    hipEvent_t after;
+    if (int rc = event(after)) return rc;
+    HIP_OK(hipEventRecord(after, s));
+    if (r->status != MH_OK || r->cancelled) {
+      r->retire = after;  // its buffers are free once the queued work has passed
+      moved = true;
+      continue;
+    }
+    HIP_OK(hipStreamWaitEvent(copy_stream, after, 0));
+    r->after = after;
     const int64_t n = r->n;
     Carver c{host_mem + r->host_off};
     r->h_prim = c.take<int32_t>(n);
     r->h_t = c.take<float>(n);
-    HIP_OK(hipMemcpyAsync(r->h_prim, r->prim.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(r->h_t, r->prim.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(r->h_prim, r->prim.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                          copy_stream));
+    HIP_OK(hipMemcpyAsync(r->h_t, r->prim.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                          copy_stream));
     for (auto& x : r->sec) {
       if (x.any) {
         r->h_occ = c.take<uint8_t>(n);
-        HIP_OK(hipMemcpyAsync(r->h_occ, x.hits.occluded, (size_t)n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(r->h_occ, x.hits.occluded, (size_t)n, hipMemcpyDeviceToHost,
+                              copy_stream));
       } else {
         r->h_bprim = c.take<int32_t>(n);
         r->h_bt = c.take<float>(n);
-        HIP_OK(hipMemcpyAsync(r->h_bprim, x.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(r->h_bt, x.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(r->h_bprim, x.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                              copy_stream));
+        HIP_OK(hipMemcpyAsync(r->h_bt, x.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                              copy_stream));
       }
     }
+    hipEvent_t copied;
+    if (int rc = event(copied)) return rc;
+    HIP_OK(hipEventRecord(copied, copy_stream));
+    r->retire = copied;
+    moved = true;
   }
-  for (int k = 0; k < ns; ++k) HIP_OK(hipStreamSynchronize(per[k]->stream));
-  {
+  // Requests whose copies (or queued work, when cancelled) are done.
+  std::vector<std::shared_ptr<Request>> retired;
+  for (auto& r : work) {
+    if (!r->retire || r->done) continue;
+    const hipError_t q = hipEventQuery(r->retire);
+    if (q == hipErrorNotReady) continue;
+    HIP_OK(q);
+    free_events.push_back(r->retire);
+    r->retire = nullptr;
+    if (r->after) free_events.push_back(r->after);
+    r->after = nullptr;
+    retired.push_back(r);
+  }
+  if (!retired.empty()) {
     std::lock_guard<std::mutex> lk(mu);
-    st.rays += traced;
-    for (auto& r : copied) {
+    for (auto& r : retired) {
+      if (r->cancelled && r->status == MH_OK) r->status = MH_E_CANCELED;
+      if (r->status != MH_OK && r->has_host) {
+        host_arena->release(r->host_off, r->host_bytes);
+        r->has_host = false;
+      }
       r->t_done = now();
       r->done = true;
       arena->release(r->dev_off, r->dev_bytes);
       active.erase(std::find(active.begin(), active.end(), r));
       finished[r->id] = r;
-      st.completed++;
+      if (r->status == MH_OK) st.completed++;
     }
+    moved = true;
+    cv_done.notify_all();
   }
-  progressed = !fresh.empty() || traced > 0 || !copied.empty();
-  if (!copied.empty()) cv_done.notify_all();
+  progressed = moved;
   return MH_OK;
 }
 
@@ -438,13 +542,13 @@ void mh_server::run() {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
       if (stop && active.empty() && pending.empty()) return;  // drain before exiting
+      if (busy_since < 0) busy_since = now();
     }
-    const double t_start = now();
-    const int rc = round();
+    const int rc = step();
     std::unique_lock<std::mutex> lk(mu);
-    st.busy_ms += now() - t_start;
     if (rc != MH_OK) {
-      // A device error ends every request in flight with that status.
+      // A device error ends every request in flight with that status (the device work
+      // queued for them is not waited for: after an error the streams are unusable).
       for (auto& r : active) {
         r->status = rc;
         r->done = true;
@@ -457,9 +561,19 @@ void mh_server::run() {
       active.clear();
       cv_done.notify_all();
     } else if (!progressed) {
-      // Finished requests that found no host space wait for collections
-      // (mh_server_wait notifies) instead of spinning.
-      cv_work.wait_for(lk, std::chrono::milliseconds(1));
+      // Nothing to queue or retire: poll the device again shortly while it has work,
+      // else wait for a new request or a collection that frees host space.
+      if (gpu_busy()) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+        lk.lock();
+      } else {
+        cv_work.wait_for(lk, std::chrono::milliseconds(1));
+      }
+    }
+    if (active.empty() && pending.empty() && busy_since >= 0) {
+      st.busy_ms += now() - busy_since;
+      busy_since = -1;
     }
     if (stop) {
       // Uncollected results are discarded at shutdown so the drain cannot stall on them.
@@ -509,12 +623,18 @@ int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_serve
                        hipHostMallocDefault));
   s->host_arena.reset(new Arena(d.host_budget));
   for (int k = 0; k < n_scenes; ++k) s->per.emplace_back(new mh_server::PerScene());
+  HIP_OK(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
   for (int k = 0; k < n_scenes; ++k) {
     auto& p = *s->per[k];
     HIP_OK(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
-    if ((rc = p.table.reserve(table_bytes())) || (rc = p.counters.reserve(mh::kBatchCounterBytes))) return rc;
+    if ((rc = p.table[0].reserve(table_bytes())) || (rc = p.table[1].reserve(table_bytes())) ||
+        (rc = p.counters.reserve(mh::kBatchCounterBytes)))
+      return rc;
     HIP_OK(hipMemset(p.counters.p, 0, mh::kBatchCounterBytes));
-    HIP_OK(hipHostMalloc(&p.host_table, table_bytes(), hipHostMallocDefault));
+    for (int t = 0; t < 2; ++t) {
+      HIP_OK(hipHostMalloc(&p.host_table[t], table_bytes(), hipHostMallocDefault));
+      HIP_OK(hipEventCreateWithFlags(&p.done[t], hipEventDisableTiming));
+    }
     p.blocks = s->launch.blocks_per_cu * mh_internal::cu_count();
     p.spill_stride = (int64_t)p.blocks * s->launch.block;
     const int64_t need = std::max<int64_t>(
@@ -602,6 +722,31 @@ int mh_server_wait(mh_server* s, uint64_t id, const mh_request_result* out,
   s->cv_work.notify_one();
   if (r->status != MH_OK) return fail(r->status, "request %llu failed", (unsigned long long)id);
   return MH_OK;
+}
+
+int mh_server_cancel(mh_server* s, uint64_t id) {
+  if (!s) return fail(MH_E_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+    if ((*it)->id != id) continue;
+    std::shared_ptr<Request> r = *it;
+    s->pending.erase(it);
+    r->status = MH_E_CANCELED;
+    r->done = true;
+    r->t_done = s->now();
+    s->finished[id] = r;
+    s->cv_done.notify_all();
+    return MH_OK;
+  }
+  for (auto& r : s->active) {
+    if (r->id != id) continue;
+    r->cancelled = true;  // the scheduler stops its chunks and frees it behind its work
+    s->cv_work.notify_one();
+    return MH_OK;
+  }
+  if (s->finished.count(id))
+    return fail(MH_E_STATE, "request %llu has already finished", (unsigned long long)id);
+  return fail(MH_E_INVALID, "unknown or already collected request %llu", (unsigned long long)id);
 }
 
 int mh_server_get_stats(mh_server* s, mh_server_stats* out) {

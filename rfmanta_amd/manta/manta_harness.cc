@@ -3,7 +3,8 @@
 // Engine/Control/RTRT.cc:568-590), then worker threads that each fill their own
 // RayPacketData with 64-ray packets and call intersect (Raytracer::traceRays,
 // Engine/Renderers/Raytracer.cc:49-53; HardShadows, Engine/Shadows/HardShadows.cc:352).
-// Exposed through one C entry point so the tests can compare it with the oracle.
+// Exposed through one C entry point so the tests can compare it with the oracle;
+// trace_seconds (may be null) receives the wall time of the intersect phase.
 
 #include <Core/Exceptions/InternalError.h>
 #include <Interface/Context.h>
@@ -14,6 +15,7 @@
 #include <Model/Primitives/WaldTriangle.h>
 
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -40,7 +42,7 @@ struct Io {
 extern "C" int mh_manta_render(const float* verts, uint32_t n_verts, const uint32_t* idx,
                                uint32_t n_tris, int prim_kind, int threads, const Io* io,
                                int64_t n, int any_hit, int frames_refit, char* err,
-                               int err_len) {
+                               int err_len, double* trace_seconds) {
   try {
     Mesh mesh;
     mesh.vertices.resize(n_verts);
@@ -80,6 +82,7 @@ extern "C" int mh_manta_render(const float* verts, uint32_t n_verts, const uint3
 
     const int64_t packets = (n + 63) / 64;
     std::atomic<int64_t> next(0);
+    const auto t_start = std::chrono::steady_clock::now();
     std::vector<std::string> errors(threads);
     std::vector<std::thread> th;
     for (int p = 0; p < threads; ++p)
@@ -125,6 +128,9 @@ extern "C" int mh_manta_render(const float* verts, uint32_t n_verts, const uint3
         }
       });
     for (auto& x : th) x.join();
+    if (trace_seconds)
+      *trace_seconds =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     for (auto& e : errors)
       if (!e.empty()) throw InternalError(e);
     return 0;

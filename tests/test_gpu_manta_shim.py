@@ -27,7 +27,7 @@ def harness():
     lib.mh_manta_render.restype = C.c_int
     lib.mh_manta_render.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int,
                                     C.c_int, C.POINTER(Io), C.c_int64, C.c_int, C.c_int,
-                                    C.c_char_p, C.c_int]
+                                    C.c_char_p, C.c_int, C.POINTER(C.c_double)]
     return lib
 
 
@@ -47,7 +47,7 @@ def render(lib, verts, idx, rays, any_hit=False, prim_kind=0, threads=8, refits=
     i = np.ascontiguousarray(idx, np.uint32)
     err = C.create_string_buffer(512)
     rc = lib.mh_manta_render(v.ctypes.data, len(v), i.ctypes.data, len(i), prim_kind, threads,
-                             C.byref(io), n, int(any_hit), refits, err, 512)
+                             C.byref(io), n, int(any_hit), refits, err, 512, None)
     assert rc == 0, err.value.decode()
     return out
 

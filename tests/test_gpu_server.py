@@ -152,3 +152,42 @@ def test_host_staging_frees_on_collection(two_scenes):
     st = srv.stats()
     assert st["completed"] == 4 and st["host_in_use"] == 0 and st["hbm_in_use"] == 0
     srv.close()
+
+
+def test_cancel_queued_and_in_flight(two_scenes):
+    """mh_server_cancel: a queued request ends at once, one in flight ends behind the work
+    already queued for it (or finishes normally if it got there first); the others are
+    unaffected and every buffer is returned."""
+    (s0, _, _), _ = two_scenes
+    cam = api.camera((0, 0.9, -1.1), (0, 0, 0), (0, 1, 0), 50.0, 128, 128)
+    n = 128 * 128
+    one = 11 * 4 * n  # primary rays + hits, no shadows (256-aligned)
+    ref = direct(s0, cam, False)
+    # one request fits the budget at a time; small chunks make many launches per request
+    srv = api.Server([s0], hbm_budget=one + 1024, chunk_rays=1024)
+    ids = [srv.submit(0, cam) for _ in range(4)]
+    srv.cancel(ids[3])  # still queued behind the budget
+    first_cancelled = True
+    try:
+        srv.cancel(ids[0])
+    except L.MhError as e:  # it had already finished
+        assert e.code == L.MH_E_STATE
+        first_cancelled = False
+    for rid in ids[1:3]:
+        o = srv.wait(rid)
+        assert np.array_equal(o["prim"], ref["prim"]) and np.array_equal(o["t"], ref["t"])
+    for rid, cancelled in ((ids[0], first_cancelled), (ids[3], True)):
+        if cancelled:
+            with pytest.raises(L.MhError) as e:
+                srv.wait(rid)
+            assert e.value.code == L.MH_E_CANCELED
+        else:
+            o = srv.wait(rid)
+            assert np.array_equal(o["prim"], ref["prim"])
+    with pytest.raises(L.MhError):  # collected
+        srv.cancel(ids[1])
+    st = srv.stats()
+    assert st["completed"] == 4 - first_cancelled - 1
+    assert st["active"] == 0 and st["pending"] == 0
+    assert st["hbm_in_use"] == 0 and st["host_in_use"] == 0
+    srv.close()
