@@ -15,7 +15,8 @@ Sections (all on synthetic inputs, generated at start-up and resident in HBM whe
   C3  request loop over two resident scenes (the C1 heightfield and an 8M-triangle soup in
       [-4,4]^3): Poisson arrivals swept over QPS, each request a random orbit camera at
       256^2 / 512^2 / 1024x768 / 1920x1080 with 1-4 spp (seed 2), primary + shadow rays;
-      p50/p99 latency from submit to results in host memory.
+      p50/p99 latency from submit to results in host memory; the offered rate doubles
+      from 25 QPS until p99 exceeds the 100 ms objective (capacity).
   C4  4000 x 1000 heightfield (7,990,002 triangles): per frame y += 0.01 sin(4x + t), GPU
       update (refit with tree rotations and leaf splits), then 1920x1080 primary + shadow +
       bounce rays through the request loop; 100 frames.
@@ -524,7 +525,7 @@ def bench_c3(api, scenes, scene_hf, args) -> dict:
     srv = api.Server([scene_hf, soup], hbm_budget=8 << 30, host_budget=2 << 30,
                      chunk_rays=1 << 21, max_active=64, shadows=True, light=scenes.C3_LIGHT)
     per_point = 64
-    qps_list = (25, 50, 100, 200, 400)
+    qps_list = (25, 50, 100, 200, 400, 800, 1600, 3200)
     reqs = scenes.c3_requests(per_point * (len(qps_list) + 1))
     # warm-up: a few requests back to back
     for sc, kw in reqs[:8]:
@@ -572,12 +573,21 @@ def bench_c3(api, scenes, scene_hf, args) -> dict:
               "achieved_qps": round(len(done) / (span_ms * 1e-3), 1)}
         points.append(pt)
         log(f"C3 {pt}")
+        if pt["p99_ms"] > C3_SLO_MS:
+            break
     st = srv.stats()
     srv.close()
     soup.close()
     ok = [p for p in points if p["p99_ms"] <= C3_SLO_MS]
     sustained = max(ok, key=lambda p: p["qps"]) if ok else None
+    over = [p for p in points if p["p99_ms"] > C3_SLO_MS]
+    # capacity: the highest offered rate whose p99 meets the objective, and the requests
+    # per second actually completed there (the sweep doubles the rate until p99 exceeds it)
+    capacity = ({"offered_qps": sustained["qps"], "achieved_qps": sustained["achieved_qps"],
+                 "p99_ms": sustained["p99_ms"], "first_over_slo_qps":
+                 min(p["qps"] for p in over) if over else None} if sustained else None)
     return {"points": points, "slo_p99_ms": C3_SLO_MS, "sustained": sustained,
+            "capacity": capacity,
             "busy_ms": round(st["busy_ms"], 1), "rays": st["rays"],
             "requests": "orbit cameras, {256^2, 512^2, 1024x768, 1920x1080} x 1-4 spp, "
                         "primary + shadow rays, 2 scenes (heightfield 1M, soup 8M)"}

@@ -134,6 +134,16 @@ int mh_scene_destroy(mh_scene* scene);
  * optional RTSAH, then upload. */
 int mh_scene_build(mh_scene* scene, const mh_build_opts* opts);
 
+/* The render-thread count (numProcs) the reference's DynBVH would see:
+ * largeSubtreeSize = records / (20 * numProcs) (DynBVH.cc:945, 1776). Sets the
+ * isLargeSubtree bits of the current tree as computeSubTreeSizes does (DynBVH.cc:941-950,
+ * 1971-1996), and of later builds (rebuild(proc, numProcs)) and updates. With several
+ * workers the reference updates seeded subtrees first and the large nodes above them
+ * after (parallelUpdateBounds, DynBVH.cc:2012-2066); every node is still refitted and
+ * rotated once after its children, so mh_scene_update's tree is the same for any count
+ * and only these bits depend on it. [1] */
+int mh_scene_set_workers(mh_scene* scene, int workers);
+
 /* The same host build without a scene or a device: DynBVH::rebuild (DynBVH.cc:1699-1793)
  * writing object_ids (n_tris ints) and 2*n_tris 32-byte BVHNode records (DynBVH.h:37-77,
  * the array size DynBVH::allocate uses, DynBVH.cc:1680-1697). *num_nodes receives the

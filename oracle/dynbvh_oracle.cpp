@@ -1670,6 +1670,63 @@ struct Updater {
     }
   }
 
+  // updateBounds<true> (DynBVH.cc:2098-2108 with bottomSubtreesDone): recurses only into
+  // children marked isLargeSubtree, then refits and rotates the node.
+  void updateBoundsLarge(int id) {
+    Node32& node = nodes[id];
+    if (node.isLeaf()) {
+      node.bounds.reset();
+      for (int i = 0; i < node.children; ++i) node.bounds.extendBox(obj_bounds[ids[node.child + i]]);
+      node.axis = 0;
+      return;
+    }
+    const int l = node.child, r = node.child + 1;
+    if (nodes[l].bits & 2) updateBoundsLarge(l);
+    if (nodes[r].bits & 2) updateBoundsLarge(r);
+    node.bounds.reset();
+    node.bounds.extendBox(nodes[l].bounds);
+    node.bounds.extendBox(nodes[r].bounds);
+    rotateNode(id);
+  }
+
+  // seedSubtrees (DynBVH.cc:1998-2010): the maximal subtrees not marked isLargeSubtree,
+  // in the order proc 0 appends them to the shared list.
+  void seedSubtrees(int id, std::vector<int>& list) {
+    if (nodes[id].bits & 2) {
+      seedSubtrees(nodes[id].child, list);
+      seedSubtrees(nodes[id].child + 1, list);
+    } else {
+      list.push_back(id);
+    }
+  }
+
+  void objectBounds() {
+    const int N = mesh.ntris;
+    obj_bounds.resize(N);
+    obj_centroids.resize(N);
+    for (int i = 0; i < N; ++i) {
+      obj_bounds[i] = triBox(mesh, i);
+      for (int a = 0; a < 3; ++a)
+        obj_centroids[i].v[a] = obj_bounds[i].hi[a] * 0.5f + obj_bounds[i].lo[a] * (1 - 0.5f);
+    }
+  }
+
+  // parallelUpdateBounds with numProcs > 1 (DynBVH.cc:2034-2065), its workers' subtree
+  // tasks run one after another as one worker popping the shared list from the back
+  // would: every seeded subtree by updateBounds<false>, then proc 0's updateBounds<true>
+  // from the root over the large nodes.
+  void updateWorkers() {
+    objectBounds();
+    std::vector<int> list;
+    seedSubtrees(0, list);
+    while (!list.empty()) {
+      const int id = list.back();
+      list.pop_back();
+      updateBounds(id);
+    }
+    updateBoundsLarge(0);
+  }
+
   // DynBVH.cc:2018-2030 then the numProcs == 1 branch
   void update() {
     const int N = mesh.ntris;
@@ -2069,6 +2126,46 @@ void oracle_update(const float* verts, int nverts, const uint32_t* idx, int ntri
   Mesh m{verts, nverts, idx, ntris};
   Updater u(m, (Node32*)nodes_v, ids, costs, ssize, *next_free, (uint32_t)(2 * (size_t)ntris / 20));
   u.update();
+}
+
+// DynBVH::update with numProcs workers (TREE_ROT): largeSubtreeSize = nodes.size() /
+// (20 * numProcs) (DynBVH.cc:945, 1776); one worker runs the numProcs == 1 branch, more run
+// the seeded-subtree schedule.
+void oracle_update_workers(const float* verts, int nverts, const uint32_t* idx, int ntris,
+                           void* nodes_v, int32_t* ids, float* costs, uint32_t* ssize,
+                           int* next_free, int num_procs) {
+  Mesh m{verts, nverts, idx, ntris};
+  const uint32_t large = (uint32_t)(2 * (size_t)ntris / (20 * (size_t)num_procs));
+  Updater u(m, (Node32*)nodes_v, ids, costs, ssize, *next_free, large);
+  if (num_procs <= 1)
+    u.update();
+  else
+    u.updateWorkers();
+}
+
+// computeSubTreeSizes (DynBVH.cc:1971-1996) with largeSubtreeSize = n_records /
+// (20 * numProcs): the isLargeSubtree bits rebuild(proc, numProcs) leaves (DynBVH.cc:1776)
+// and preprocess sets on a loaded tree (DynBVH.cc:941-950).
+void oracle_mark_large(void* nodes_v, int n_records, int num_procs) {
+  Node32* nodes = (Node32*)nodes_v;
+  const uint32_t large = (uint32_t)((size_t)n_records / (20 * (size_t)num_procs));
+  struct Marks {
+    Node32* n;
+    uint32_t large;
+    uint32_t go(int id) {
+      if (n[id].isLeaf()) {
+        n[id].bits &= ~2;
+        return 1;
+      }
+      const uint32_t sz = go(n[id].child) + go(n[id].child + 1) + 1;
+      if (sz > large)
+        n[id].bits |= 2;
+      else
+        n[id].bits &= ~2;
+      return sz;
+    }
+  } marks{nodes, large};
+  marks.go(0);
 }
 
 // Wald per-triangle constants (11 floats + k as float) for tests of the setup pass.

@@ -61,6 +61,10 @@ def load() -> C.CDLL:
         lib.oracle_rot_init.argtypes = [P, C.c_int, P, P]
         lib.oracle_update.restype = None
         lib.oracle_update.argtypes = [P, C.c_int, P, C.c_int, P, P, P, P, P]
+        lib.oracle_update_workers.restype = None
+        lib.oracle_update_workers.argtypes = [P, C.c_int, P, C.c_int, P, P, P, P, P, C.c_int]
+        lib.oracle_mark_large.restype = None
+        lib.oracle_mark_large.argtypes = [P, C.c_int, C.c_int]
         lib.oracle_wald_setup.restype = None
         lib.oracle_wald_setup.argtypes = [P, P, C.c_int, P]
         lib.oracle_node_size.restype = C.c_int
@@ -216,11 +220,23 @@ class RotState:
         self.next_free = C.c_int(int(num_nodes))
         load().oracle_rot_init(_p(self.nodes), n, _p(self.costs), _p(self.ssize))
 
-    def update(self, verts, idx):
-        """One DynBVH::update with rotations and leaf splits, in place."""
+    def update(self, verts, idx, workers=None):
+        """One DynBVH::update with rotations and leaf splits, in place. workers=None runs
+        the one-thread update; workers=N the N-worker schedule with its large-subtree
+        threshold."""
         v, i = _mesh(verts, idx)
-        load().oracle_update(_p(v), len(v), _p(i), len(i), _p(self.nodes), _p(self.ids),
-                             _p(self.costs), _p(self.ssize), C.byref(self.next_free))
+        if workers is None:
+            load().oracle_update(_p(v), len(v), _p(i), len(i), _p(self.nodes), _p(self.ids),
+                                 _p(self.costs), _p(self.ssize), C.byref(self.next_free))
+        else:
+            load().oracle_update_workers(_p(v), len(v), _p(i), len(i), _p(self.nodes),
+                                         _p(self.ids), _p(self.costs), _p(self.ssize),
+                                         C.byref(self.next_free), int(workers))
+
+
+def mark_large(nodes, workers):
+    """isLargeSubtree bits for `workers` render threads (in place)."""
+    load().oracle_mark_large(_p(nodes), len(nodes), int(workers))
 
 
 def ttp_probs(parent, left, right):

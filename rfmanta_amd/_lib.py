@@ -159,6 +159,7 @@ _SIGS = {
     "mh_scene_set_vertices": (C.c_int, [_P, _P, C.c_int, _P]),
     "mh_scene_refit": (C.c_int, [_P, _P]),
     "mh_scene_update": (C.c_int, [_P, _P]),
+    "mh_scene_set_workers": (C.c_int, [_P, C.c_int]),
     "mh_scene_vertices_device": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "mh_scene_displace_sine": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P]),
     "mh_trace": (C.c_int, [_P, C.POINTER(Rays), C.POINTER(Hits), C.c_int64, C.c_uint32,

@@ -253,6 +253,10 @@ class Scene:
         L.check(L.load().mh_scene_set_vertices(self.h, v.ctypes.data, 0, stream))
         self.verts = v.reshape(-1, 3)
 
+    def set_workers(self, workers: int) -> None:
+        """Render threads of the Manta instance (numProcs): the isLargeSubtree threshold."""
+        L.check(L.load().mh_scene_set_workers(self.h, int(workers)))
+
     def refit(self, stream=None) -> None:
         """Bounds-only refit (no rotations)."""
         L.check(L.load().mh_scene_refit(self.h, stream))

@@ -96,6 +96,10 @@ struct mh_scene {
   bool updatable = false;
   uint32_t n_inner = 0, n_leaves = 0, depth = 0;
   uint32_t n_records = 0;  // records held (2T after a build; the file's count after a load)
+  // Render threads of the Manta instance (numProcs): largeSubtreeSize = records / (20 *
+  // workers) (DynBVH.cc:945, 1776) decides the isLargeSubtree bits.
+  int workers = 1;
+  uint32_t large_limit() const { return (uint32_t)(n_records / (20 * (uint32_t)workers)); }
 
   DevBuf refs, pairs, tris, ids, big, d_verts, d_base, d_idx, leaves, levels;
   // rotating update (mh_scene_update): costs, subtree sizes, parents, object boxes and
@@ -106,11 +110,16 @@ struct mh_scene {
   std::vector<int64_t> level_off;  // inner records of depth d: levels[level_off[d], level_off[d+1])
 
   std::mutex spill_mu;  // guards the per-stream spill areas
-  std::mutex stage_mu;  // serialises mh_trace_host calls on the staging buffer
+  std::mutex stage_mu;  // serialises mh_trace_host calls on the staging buffers
   std::map<void*, DevBuf*> spill;
-  DevBuf stage;
+  DevBuf stage;                       // mh_trace_host: device staging
+  void* host_stage = nullptr;         // mh_trace_host: pinned host staging
+  size_t host_stage_bytes = 0;
+  hipStream_t stage_stream = nullptr;  // mh_trace_host: the scene's own stream
   ~mh_scene() {
     for (auto& kv : spill) delete kv.second;
+    if (host_stage) (void)hipHostFree(host_stage);
+    if (stage_stream) (void)hipStreamDestroy(stage_stream);
   }
   size_t device_bytes() const {
     return refs.bytes + pairs.bytes + tris.bytes + ids.bytes + big.bytes + d_verts.bytes +

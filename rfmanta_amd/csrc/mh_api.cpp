@@ -495,7 +495,7 @@ int build_device(mh_scene* s, int rtsah) {
   d.u.err = reinterpret_cast<int*>(cnt + 1);
   d.u.n_records = (int32_t)nrec;
   d.u.n_objects = (int32_t)T;
-  d.u.large = (uint32_t)(nrec / 20);
+  d.u.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
   d.slots = slots.as<float>();
   d.verts = s->d_verts.as<float>();
   d.n_verts = s->nv;
@@ -586,7 +586,7 @@ int build_device(mh_scene* s, int rtsah) {
   const uint32_t n_final = (uint32_t)(level_off.back() + n_leaves);
   f.n_records = (int32_t)nrec;
   f.n_final = (int32_t)n_final;
-  f.large = (uint32_t)(nrec / 20);
+  f.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
   f.rtsah = rtsah;
   const int n_levels = (int)level_off.size() - 1;
   HIP_OK((hipError_t)mh::build_launch_ranks(f, level_off.data(), n_levels, n_leaves, st));
@@ -639,6 +639,7 @@ int mh_scene_build(mh_scene* s, const mh_build_opts* opts) {
   }
   try {
     s->bvh = mh::build_dynbvh(s->verts.data(), s->nv, s->idx.data(), s->nt, threads);
+    if (s->workers != 1) mh::mark_large_subtrees(s->bvh.nodes, s->workers);
     if (rtsah) mh::compute_rtsah(s->bvh.nodes, threads);
   } catch (const std::bad_alloc&) {
     return fail(MH_E_NOMEM, "host memory for the SAH build");
@@ -647,6 +648,51 @@ int mh_scene_build(mh_scene* s, const mh_build_opts* opts) {
   }
   s->prim_kind = kind;
   return upload(s);
+}
+
+int mh_scene_set_workers(mh_scene* s, int workers) {
+  if (!s) return fail(MH_E_INVALID, "null scene");
+  if (workers < 1 || workers > 4096) return fail(MH_E_INVALID, "workers %d not in 1..4096", workers);
+  s->workers = workers;
+  if (!s->built) return MH_OK;  // applies to the next build
+  int rc = ensure_init();
+  if (rc) return rc;
+  // computeSubTreeSizes over the current tree with the new threshold (DynBVH.cc:941-950,
+  // 1971-1996): reachable records only, sizes counted in records.
+  const uint32_t nn = s->n_records;
+  std::vector<mh::RefNode> nodes(nn);
+  HIP_OK(hipMemcpy(nodes.data(), s->refs.p, (size_t)nn * sizeof(mh::RefNode),
+                   hipMemcpyDeviceToHost));
+  const uint32_t limit = s->large_limit();
+  std::vector<uint32_t> size(nn, 0);
+  std::vector<std::pair<int32_t, bool>> st{{0, false}};
+  while (!st.empty()) {
+    auto [id, done] = st.back();
+    st.pop_back();
+    if (id < 0 || (uint32_t)id >= nn) return fail(MH_E_STATE, "record %d outside the tree", id);
+    mh::RefNode& n = nodes[id];
+    if (n.leaf()) {
+      size[id] = 1;
+      n.bits &= (uint8_t)~2;
+      continue;
+    }
+    if (!done) {
+      if ((uint32_t)n.child + 1 >= nn) return fail(MH_E_STATE, "record %d: child outside", id);
+      st.push_back({id, true});
+      st.push_back({n.child + 1, false});
+      st.push_back({n.child, false});
+      continue;
+    }
+    size[id] = size[n.child] + size[n.child + 1] + 1;
+    if (size[id] > limit)
+      n.bits |= 2;
+    else
+      n.bits &= (uint8_t)~2;
+  }
+  HIP_OK(hipMemcpy(s->refs.p, nodes.data(), (size_t)nn * sizeof(mh::RefNode),
+                   hipMemcpyHostToDevice));
+  if (s->bvh.nodes.size() == nodes.size()) s->bvh.nodes = nodes;
+  return MH_OK;
 }
 
 int mh_dynbvh_build_host(const float* verts, uint32_t n_verts, const uint32_t* idx,
@@ -869,7 +915,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   u.err = err;
   u.n_records = (int32_t)nrec;
   u.n_objects = (int32_t)T;
-  u.large = (uint32_t)(nrec / 20);  // largeSubtreeSize = nodes.size() / 20 (DynBVH.cc:1776)
+  // largeSubtreeSize = nodes.size() / (20 * numProcs) (DynBVH.cc:1776)
+  u.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
   HIP_OK((hipError_t)mh::launch_obj_bounds(u, st));
   HIP_OK((hipError_t)mh::launch_update_tree(u, s->leaves.as<int32_t>(), s->n_leaves,
                                             s->levels.as<int32_t>(), s->level_off.data(),
@@ -1025,6 +1072,9 @@ int mh_trace(const mh_scene* s, const mh_rays* rays, const mh_hits* hits, int64_
   return trace_impl(s, rays, hits, n, flags, opts, stream);
 }
 
+// Ray count up to which mh_trace_host packs rays and results into pinned host memory.
+constexpr int64_t kPinnedStageRays = int64_t(1) << 16;
+
 int mh_trace_host(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int64_t n,
                   uint32_t flags, const mh_trace_opts* opts) {
   mh_scene* s = const_cast<mh_scene*>(cs);
@@ -1033,25 +1083,75 @@ int mh_trace_host(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, 
   int rc = ensure_init();
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(s->stage_mu);
-  // Staging layout: 7 float arrays + mask bytes in, 4 result arrays + occluded bytes out.
+  // One staging layout in device memory (and, for small sets, in pinned host memory): 7
+  // float arrays and the mask bytes in, 4 result arrays and the occluded bytes out, on the
+  // scene's own stream.
   const size_t f = (size_t)n * 4, al = 256;
   auto up = [&](size_t x) { return (x + al - 1) / al * al; };
   const size_t in_bytes = 7 * up(f) + up((size_t)n);
   const size_t out_bytes = 4 * up(f) + up((size_t)n);
   if ((rc = s->stage.reserve(in_bytes + out_bytes))) return rc;
+  // A blocking stream: work the caller queued on the null stream (a refit, an upload)
+  // comes first, as it did for the null-stream copies of this entry point.
+  if (!s->stage_stream) HIP_OK(hipStreamCreateWithFlags(&s->stage_stream, hipStreamDefault));
+  hipStream_t st = s->stage_stream;
   char* base = s->stage.as<char>();
-  float* d[7];
+  if (n > kPinnedStageRays) {
+    // Large sets: the runtime's own pipelined copies from pageable memory, array by array.
+    const float* src[7] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+    float* d[7];
+    for (int i = 0; i < 7; ++i) {
+      if (!src[i]) return fail(MH_E_INVALID, "ray array %d is null", i);
+      d[i] = reinterpret_cast<float*>(base + i * up(f));
+      HIP_OK(hipMemcpyAsync(d[i], src[i], f, hipMemcpyHostToDevice, st));
+    }
+    uint8_t* dm = nullptr;
+    if (rays->masked) {
+      dm = reinterpret_cast<uint8_t*>(base + 7 * up(f));
+      HIP_OK(hipMemcpyAsync(dm, rays->masked, (size_t)n, hipMemcpyHostToDevice, st));
+    }
+    char* ob = base + in_bytes;
+    mh_hits dh{};
+    dh.prim = hits->prim ? reinterpret_cast<int32_t*>(ob) : nullptr;
+    dh.t = hits->t ? reinterpret_cast<float*>(ob + up(f)) : nullptr;
+    dh.u = hits->u ? reinterpret_cast<float*>(ob + 2 * up(f)) : nullptr;
+    dh.v = hits->v ? reinterpret_cast<float*>(ob + 3 * up(f)) : nullptr;
+    dh.occluded = hits->occluded ? reinterpret_cast<uint8_t*>(ob + 4 * up(f)) : nullptr;
+    mh_rays dr{d[0], d[1], d[2], d[3], d[4], d[5], d[6], dm};
+    if ((rc = trace_impl(s, &dr, &dh, n, flags, opts, st))) return rc;
+    if (hits->prim) HIP_OK(hipMemcpyAsync(hits->prim, dh.prim, f, hipMemcpyDeviceToHost, st));
+    if (hits->t) HIP_OK(hipMemcpyAsync(hits->t, dh.t, f, hipMemcpyDeviceToHost, st));
+    if (hits->u) HIP_OK(hipMemcpyAsync(hits->u, dh.u, f, hipMemcpyDeviceToHost, st));
+    if (hits->v) HIP_OK(hipMemcpyAsync(hits->v, dh.v, f, hipMemcpyDeviceToHost, st));
+    if (hits->occluded)
+      HIP_OK(hipMemcpyAsync(hits->occluded, dh.occluded, (size_t)n, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return MH_OK;
+  }
+  // Small sets (the packets of a render thread gathered by HipDynBVH): packed into pinned
+  // host memory, one copy each way.
+  if (s->host_stage_bytes < in_bytes + out_bytes) {
+    if (s->host_stage) (void)hipHostFree(s->host_stage);
+    s->host_stage = nullptr;
+    s->host_stage_bytes = 0;
+    HIP_OK(hipHostMalloc(&s->host_stage, in_bytes + out_bytes, hipHostMallocDefault));
+    s->host_stage_bytes = in_bytes + out_bytes;
+  }
+  char* hbase = static_cast<char*>(s->host_stage);
   const float* src[7] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmax};
+  float* d[7];
   for (int i = 0; i < 7; ++i) {
     if (!src[i]) return fail(MH_E_INVALID, "ray array %d is null", i);
+    std::memcpy(hbase + i * up(f), src[i], f);
     d[i] = reinterpret_cast<float*>(base + i * up(f));
-    HIP_OK(hipMemcpy(d[i], src[i], f, hipMemcpyHostToDevice));
   }
   uint8_t* dm = nullptr;
   if (rays->masked) {
+    std::memcpy(hbase + 7 * up(f), rays->masked, (size_t)n);
     dm = reinterpret_cast<uint8_t*>(base + 7 * up(f));
-    HIP_OK(hipMemcpy(dm, rays->masked, (size_t)n, hipMemcpyHostToDevice));
   }
+  HIP_OK(hipMemcpyAsync(base, hbase, rays->masked ? in_bytes : 7 * up(f), hipMemcpyHostToDevice,
+                        st));
   char* ob = base + in_bytes;
   mh_hits dh{};
   dh.prim = hits->prim ? reinterpret_cast<int32_t*>(ob) : nullptr;
@@ -1060,14 +1160,15 @@ int mh_trace_host(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, 
   dh.v = hits->v ? reinterpret_cast<float*>(ob + 3 * up(f)) : nullptr;
   dh.occluded = hits->occluded ? reinterpret_cast<uint8_t*>(ob + 4 * up(f)) : nullptr;
   mh_rays dr{d[0], d[1], d[2], d[3], d[4], d[5], d[6], dm};
-  if ((rc = trace_impl(s, &dr, &dh, n, flags, opts, nullptr))) return rc;
-  HIP_OK(hipDeviceSynchronize());
-  if (hits->prim) HIP_OK(hipMemcpy(hits->prim, dh.prim, f, hipMemcpyDeviceToHost));
-  if (hits->t) HIP_OK(hipMemcpy(hits->t, dh.t, f, hipMemcpyDeviceToHost));
-  if (hits->u) HIP_OK(hipMemcpy(hits->u, dh.u, f, hipMemcpyDeviceToHost));
-  if (hits->v) HIP_OK(hipMemcpy(hits->v, dh.v, f, hipMemcpyDeviceToHost));
-  if (hits->occluded)
-    HIP_OK(hipMemcpy(hits->occluded, dh.occluded, (size_t)n, hipMemcpyDeviceToHost));
+  if ((rc = trace_impl(s, &dr, &dh, n, flags, opts, st))) return rc;
+  HIP_OK(hipMemcpyAsync(hbase + in_bytes, ob, out_bytes, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  const char* ho = hbase + in_bytes;
+  if (hits->prim) std::memcpy(hits->prim, ho, f);
+  if (hits->t) std::memcpy(hits->t, ho + up(f), f);
+  if (hits->u) std::memcpy(hits->u, ho + 2 * up(f), f);
+  if (hits->v) std::memcpy(hits->v, ho + 3 * up(f), f);
+  if (hits->occluded) std::memcpy(hits->occluded, ho + 4 * up(f), (size_t)n);
   return MH_OK;
 }
 

@@ -360,8 +360,8 @@ void compute_rtsah(std::vector<RefNode>& nodes, int threads) {
   pass.run();
 }
 
-void mark_large_subtrees(std::vector<RefNode>& nodes) {
-  const unsigned limit = (unsigned)(nodes.size() / 20);
+void mark_large_subtrees(std::vector<RefNode>& nodes, int workers) {
+  const unsigned limit = (unsigned)(nodes.size() / (20 * (size_t)(workers > 0 ? workers : 1)));
   // Post-order over an explicit stack: subtree record counts.
   std::vector<unsigned> size(nodes.size(), 0);
   std::vector<std::pair<int, bool>> st;

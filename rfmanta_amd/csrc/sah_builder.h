@@ -40,7 +40,8 @@ DynBvh build_dynbvh(const float* verts, uint32_t n_verts, const uint32_t* idx, u
 // isLeftCheaper for every inner node (DynBVH::computeTraversalCost, DynBVH.cc:1583-1678).
 void compute_rtsah(std::vector<RefNode>& nodes, int threads);
 
-// isLargeSubtree with largeSubtreeSize = nodes.size()/20 (DynBVH.cc:1776, 1971-1996).
-void mark_large_subtrees(std::vector<RefNode>& nodes);
+// isLargeSubtree with largeSubtreeSize = nodes.size() / (20 * workers) (DynBVH.cc:1776,
+// 1971-1996); workers is the reference's numProcs.
+void mark_large_subtrees(std::vector<RefNode>& nodes, int workers = 1);
 
 }  // namespace mh

@@ -125,7 +125,8 @@ class HipDynBVH::PacketBatcher {
 
 HipDynBVH::HipDynBVH()
     : group_(0), mesh_(0), group_changed_(false), scene_(0), prim_kind_(MH_PRIM_KENSLER_SHIRLEY),
-      max_packets_(256), wait_us_(200), barrier_count_(0), barrier_generation_(0) {}
+      workers_(0), max_packets_(256), wait_us_(200), barrier_count_(0),
+      barrier_generation_(0) {}
 
 HipDynBVH::~HipDynBVH() {
   batcher_.reset();
@@ -164,7 +165,7 @@ void HipDynBVH::preprocess(const PreprocessContext& context) {
 // DynBVH::rebuild / update (DynBVH.cc:1574-1581, 1699-1793): every render thread calls
 // in; thread 0 does the work and the others wait for it.
 void HipDynBVH::rebuild(int proc, int numProcs) {
-  if (proc == 0) rebuildSerial();
+  if (proc == 0) rebuildSerial(numProcs);
   rendezvous(numProcs);
 }
 
@@ -199,9 +200,13 @@ void HipDynBVH::ensureScene() {
                                                                  : MH_PRIM_KENSLER_SHIRLEY;
 }
 
-void HipDynBVH::rebuildSerial() {
+void HipDynBVH::rebuildSerial(int numProcs) {
   if (group_changed_ || !scene_) {
     ensureScene();
+    // rebuild(proc, numProcs): largeSubtreeSize = nodes.size() / (20 * numProcs)
+    // (DynBVH.cc:1776) before the build
+    if (mh_scene_set_workers(scene_, numProcs) != MH_OK) fail("rebuild");
+    workers_ = numProcs;
     mh_build_opts o{prim_kind_, 1, 0, 0};
     if (mh_scene_build(scene_, &o) != MH_OK) fail("rebuild");
     group_changed_ = false;
@@ -209,8 +214,15 @@ void HipDynBVH::rebuildSerial() {
     if (needToSaveFile) saveToFile(saveFileName);
     return;
   }
+  // A loaded tree, or a new thread count: computeSubTreeSizes with this numProcs, as
+  // preprocess does before an update (DynBVH.cc:941-950).
+  if (workers_ != numProcs) {
+    if (mh_scene_set_workers(scene_, numProcs) != MH_OK) fail("update");
+    workers_ = numProcs;
+  }
   // Same group, moved vertices: DynBVH::update on the device, i.e. the bounds refit with
-  // tree rotations and leaf splits (DynBVH.cc:1574-1581, 2012-2036 with numProcs == 1).
+  // tree rotations and leaf splits (DynBVH.cc:1574-1581, 2012-2066; the tree does not
+  // depend on numProcs, only the isLargeSubtree bits do).
   const float* verts = &mesh_->vertices[0].data[0];
   if (mh_scene_set_vertices(scene_, verts, 0, 0) != MH_OK) fail("update");
   if (mh_scene_update(scene_, 0) != MH_OK) fail("update");
@@ -222,6 +234,7 @@ bool HipDynBVH::buildFromFile(const std::string& file) {
   ensureScene();
   if (mh_scene_load_dynbvh_file(scene_, file.c_str(), prim_kind_) != MH_OK) return false;
   group_changed_ = false;
+  workers_ = 0;  // preprocess sets the bits for its numProcs
   batcher_.reset(new PacketBatcher(scene_, max_packets_, wait_us_));
   return true;
 }

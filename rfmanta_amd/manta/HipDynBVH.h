@@ -55,7 +55,7 @@ class HipDynBVH : public AccelerationStructure {
  private:
   class PacketBatcher;
 
-  void rebuildSerial();
+  void rebuildSerial(int numProcs);
   void ensureScene();
   void rendezvous(int numProcs);
   [[noreturn]] void fail(const char* what) const;
@@ -65,6 +65,7 @@ class HipDynBVH : public AccelerationStructure {
   bool group_changed_;
   mh_scene* scene_;
   int prim_kind_;
+  int workers_;  // numProcs the scene's isLargeSubtree bits were set for (0: none yet)
   std::unique_ptr<PacketBatcher> batcher_;
   int max_packets_, wait_us_;
 

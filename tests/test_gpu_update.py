@@ -193,3 +193,38 @@ def test_update_wald_records(oracle):
             o = oracle.trace(vd, i, st.nodes, st.ids, rays, packet_size=1, prim_kind=1)
             assert np.array_equal(g["prim"], o["prim"]), f
             assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32)), f
+
+
+def test_update_with_workers_matches_oracle(oracle):
+    """mh_scene_set_workers(N): the isLargeSubtree bits a Manta instance with N render
+    threads has after rebuild (DynBVH.cc:1776) and after every update of the N-worker
+    schedule (DynBVH.cc:2034-2065, restated by the oracle), on a soup whose leaves split;
+    a later change of the count re-marks the current tree (DynBVH.cc:941-950)."""
+    v, i = scenes.soup(6000, seed=7, sigma=0.05)
+    W = 8
+    nodes, ids, used = oracle.build_bvh(v, i)
+    want = nodes.copy()
+    oracle.mark_large(want, W)
+    st = oracle.RotState(want, ids, used)
+    with api.Scene(v, i) as s:
+        s.set_workers(W)
+        s.build()
+        gi, gn = s.export_dynbvh()
+        assert np.array_equal(gi, ids) and gn.tobytes() == want.tobytes()
+        for f in range(4):
+            vd = (v + np.random.default_rng(f).normal(0, 0.3, v.shape).astype(np.float32)
+                  * (f + 1) / 5).astype(np.float32)
+            s.set_vertices(vd)
+            s.update()
+            api.sync()
+            st.update(vd, i, workers=W)
+            gi, gn = s.export_dynbvh()
+            ok, msg, _ = canonical_equal_levels(gn, gi, st.nodes, st.ids)
+            assert ok, (f, msg)
+        assert st.next_free.value > used
+        s.set_workers(1)
+        gi, gn = s.export_dynbvh()
+        one = st.nodes.copy()
+        oracle.mark_large(one, 1)
+        ok, msg, _ = canonical_equal_levels(gn, gi, one, st.ids)
+        assert ok, msg
