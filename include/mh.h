@@ -88,6 +88,9 @@ typedef struct mh_hits {
  * and visit order. */
 #define MH_LAYOUT_PAIR 0 /* 64-byte nodes holding both child boxes */
 #define MH_LAYOUT_REF 1  /* the reference's 32-byte BVHNode records (frames only) */
+#define MH_LAYOUT_SOA 2  /* the 64-byte nodes as four arrays of their 16-byte fields (frames
+                            only; the scene keeps the copy, refreshed after builds, refits
+                            and updates) */
 
 /* Trace options; NULL means the defaults given in brackets. */
 typedef struct mh_trace_opts {

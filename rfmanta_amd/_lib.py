@@ -72,6 +72,7 @@ class LaunchConfig(C.Structure):
 
 MH_LAYOUT_PAIR = 0
 MH_LAYOUT_REF = 1
+MH_LAYOUT_SOA = 2
 
 
 class RayBuffers(C.Structure):

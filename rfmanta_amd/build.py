@@ -33,7 +33,8 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt
 HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}"]
 
 HIP_SOURCES = ["trace.hip", "batch.hip", "batch_ks_pair.hip", "batch_ks_ref.hip",
-               "batch_wald_pair.hip", "batch_wald_ref.hip", "refit.hip", "rotate.hip",
+               "batch_ks_soa.hip", "batch_wald_pair.hip", "batch_wald_ref.hip",
+               "batch_wald_soa.hip", "refit.hip", "rotate.hip",
                "build.hip", "shade.hip", "genrays.hip", "util.hip", "noalloc.hip"]
 HOST_SOURCES = ["sah_builder.cpp", "mh_api.cpp", "frame_api.cpp", "rays_api.cpp", "server.cpp",
                 "ply.cpp", "obj.cpp"]

@@ -10,11 +10,14 @@ int launch_trace_batch(const DevScene& sc, const BatchLaunch& bl, void* stream) 
     return (int)hipErrorInvalidValue;
   if (bl.kinds < kKindMixed || bl.kinds > kKindAny) return (int)hipErrorInvalidValue;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (bl.layout == kLayoutSoa && !sc.pairs_soa) return (int)hipErrorInvalidValue;
   if (sc.prim_kind == 1)
-    return bl.layout == kLayoutRef ? launch_batch_wald_ref(sc, bl, s)
-                                   : launch_batch_wald_pair(sc, bl, s);
-  return bl.layout == kLayoutRef ? launch_batch_ks_ref(sc, bl, s)
-                                 : launch_batch_ks_pair(sc, bl, s);
+    return bl.layout == kLayoutRef   ? launch_batch_wald_ref(sc, bl, s)
+           : bl.layout == kLayoutSoa ? launch_batch_wald_soa(sc, bl, s)
+                                     : launch_batch_wald_pair(sc, bl, s);
+  return bl.layout == kLayoutRef   ? launch_batch_ks_ref(sc, bl, s)
+         : bl.layout == kLayoutSoa ? launch_batch_ks_soa(sc, bl, s)
+                                   : launch_batch_ks_pair(sc, bl, s);
 }
 
 }  // namespace mh

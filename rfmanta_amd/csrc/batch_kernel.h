@@ -11,16 +11,18 @@
 //     per C rays, which balances the load between waves;
 //   static stride (refill 64, chunk 0): the same without the counter;
 //   per-lane refill (refill < 64): each wave keeps a pool of ray numbers taken C at a
-//     time from the counter; a lane whose ray has finished takes the next number as soon
-//     as at least `refill` lanes of its wave are idle, so lanes do not wait for the
-//     wave's slowest ray (ballot + popcount hand-out, no LDS traffic).
+//     time from the counter (or from its XCD's eighth of the rays first); a lane whose
+//     ray has finished takes the next number, between traversal rounds, as soon as at
+//     least `refill` lanes of its wave are idle, so lanes do not wait for the wave's
+//     slowest ray (ballot + popcount hand-out, no LDS traffic).
 // The last wave to leave resets the counters, so consecutive launches (and graph
 // replays) need no clearing pass.
 //
-// Two node layouts, both giving the reference's one-ray results and visit order
+// Three node layouts, all giving the reference's one-ray results and visit order
 // (DynBVH::intersectNode on a one-ray packet, Model/Groups/DynBVH.cc:274-404; the
-// traversal loops are traverse.h's): PAIR (64-byte records with both child boxes) and REF
-// (the reference's 32-byte records).
+// traversal loops are traverse.h's): PAIR (64-byte records with both child boxes), SOA
+// (the same records as four arrays of their 16-byte fields) and REF (the reference's
+// 32-byte records).
 // Triangle tests and arithmetic are those of trace.hip (shared header).
 //
 // A table whose batches are all closest-hit (or all any-hit) runs a kernel specialised to
@@ -77,7 +79,7 @@ __device__ __forceinline__ void trace_span(const DevScene& sc, const RefNode& ro
   }
 }
 
-// Per-lane state of the lane-refill schedule: one traversal advanced a node at a time.
+// Per-lane state of the lane-refill schedule: one traversal advanced a round at a time.
 struct Lane {
   Ray r;
   RayHit h;
@@ -114,63 +116,20 @@ __device__ __forceinline__ bool step_ref(const DevScene& sc, Lane& L, LaneStack<
   return true;
 }
 
-// Per-lane refill schedule, PAIR layout: one step of traverse.h's trace_pair loop (an inner
-// node, or a leaf and the pops after it), resumed from the lane's state, so that the wave
-// can refill lanes whose rays have finished between any two steps. Returns false when the
-// ray is finished.
-template <bool ANY, int PRIM, int S, int BLOCK, bool FAST>
-__device__ __forceinline__ bool step_pair(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
-  unsigned long long ntri = 0;
-  if (L.cur >= 0) {
-    const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
-    const float4 A = fetch_record(pt, L.cur, 0), B = fetch_record(pt, L.cur, 16),
-                 C = fetch_record(pt, L.cur, 32), Mf = fetch_record(pt, L.cur, 48);
-    const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
-                             __float_as_int(Mf.w));
-    float tn0, tn1;
-    const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
-    const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
-    const bool one_first = ANY ? ((M.z >> 2) & 1) == 0 : __builtin_amdgcn_ubfe(L.r.dpos, M.z, 1) == 0;
-    if (h0 && h1) st.push(one_first ? M.x : M.y, one_first ? tn0 : tn1);
-    if (h0 || h1) {
-      L.cur = (h1 && (one_first || !h0)) ? M.y : M.x;
-      return true;
-    }
-  } else {
-    const int code = ~L.cur;
-    const int first = code >> kLeafCountBits;
-    int cnt = code & kLeafCountInline;
-    if (cnt == kLeafCountInline) cnt = sc.big_count[first];
-    if (leaf<ANY, PRIM, false>(sc, first, cnt, L.r, L.h, ntri)) return false;
-  }
-  while (!st.empty()) {
-    int ref;
-    float t;
-    st.pop(ref, t);
-    if (t <= L.h.minT) {
-      L.cur = ref;
-      return true;
-    }
-  }
-  return false;
-}
-
-// Per-lane refill schedule, PAIR layout: one round of traverse.h's trace_pair loop, resumed
-// from the lane's state: the walk down inner nodes to a leaf (or to a miss), the leaf's
+// Per-lane refill schedule, PAIR and SOA layouts: one round of traverse.h's trace_pair loop,
+// resumed from the lane's state: the walk down inner nodes to a leaf (or to a miss), the leaf's
 // triangles and the pops to the next entry still worth visiting. The lanes of a wave walk
 // the inner nodes in lockstep (a lane that reached its leaf waits for the others), so the
 // wave issues each inner-node step once for all of them, as the plain kernel does; a
 // finished lane is refilled between rounds. Returns false when the ray is finished.
-template <bool ANY, int PRIM, int S, int BLOCK, bool FAST>
+template <bool ANY, int PRIM, int S, int BLOCK, bool FAST, int LAYOUT>
 __device__ __forceinline__ bool round_pair(const DevScene& sc, Lane& L, LaneStack<S, BLOCK>& st) {
   unsigned long long ntri = 0;
-  const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
   bool missed = false;
   while (L.cur >= 0) {
-    const float4 A = fetch_record(pt, L.cur, 0), B = fetch_record(pt, L.cur, 16),
-                 C = fetch_record(pt, L.cur, 32), Mf = fetch_record(pt, L.cur, 48);
-    const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
-                             __float_as_int(Mf.w));
+    float4 A, B, C;
+    int4 M;
+    fetch_pair<LAYOUT>(sc, L.cur, A, B, C, M);
     float tn0, tn1;
     const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, L.r, L.h.minT, tn0);
     const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, L.r, L.h.minT, tn1);
@@ -254,7 +213,7 @@ __device__ __forceinline__ bool start_in(const DevScene& sc, const RefNode& root
   L.h = RayHit{B.rays.tmax[L.li], -1, 0.f, 0.f};
   L.regular = regular_ray(L.r, L.h.minT);  // minT only falls to finite hit distances
   st.reset();
-  if (LAYOUT == kLayoutPair) {
+  if (LAYOUT != kLayoutRef) {
     float tn;
     if (!slab(root.box.lo[0], root.box.hi[0], root.box.lo[1], root.box.hi[1], root.box.lo[2],
               root.box.hi[2], L.r, L.h.minT, tn)) {
@@ -408,23 +367,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
           active = start_ray<PRIM, LAYOUT, S, BLOCK>(sc, root, batches, s_start, nb, g, L, st);
         }
       }
-      if constexpr (LAYOUT == kLayoutPair) {
+      if constexpr (LAYOUT != kLayoutRef) {
         // min/max slab test when every ray of the wave allows it (wave-uniform choice)
         const bool fast = __ballot(active && !L.regular) == 0ull;
         if (active) {
           bool more;
           if (KIND == kKindClosest)
-            more = fast ? round_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
-                        : round_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<false, PRIM, S, BLOCK, true, LAYOUT>(sc, L, st)
+                        : round_pair<false, PRIM, S, BLOCK, false, LAYOUT>(sc, L, st);
           else if (KIND == kKindAny)
-            more = fast ? round_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
-                        : round_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<true, PRIM, S, BLOCK, true, LAYOUT>(sc, L, st)
+                        : round_pair<true, PRIM, S, BLOCK, false, LAYOUT>(sc, L, st);
           else if (L.any)
-            more = fast ? round_pair<true, PRIM, S, BLOCK, true>(sc, L, st)
-                        : round_pair<true, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<true, PRIM, S, BLOCK, true, LAYOUT>(sc, L, st)
+                        : round_pair<true, PRIM, S, BLOCK, false, LAYOUT>(sc, L, st);
           else
-            more = fast ? round_pair<false, PRIM, S, BLOCK, true>(sc, L, st)
-                        : round_pair<false, PRIM, S, BLOCK, false>(sc, L, st);
+            more = fast ? round_pair<false, PRIM, S, BLOCK, true, LAYOUT>(sc, L, st)
+                        : round_pair<false, PRIM, S, BLOCK, false, LAYOUT>(sc, L, st);
           if (!more) {
             active = false;
             pending = true;
@@ -511,5 +470,7 @@ int launch_batch_ks_pair(const DevScene& sc, const BatchLaunch& bl, hipStream_t 
 int launch_batch_ks_ref(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 int launch_batch_wald_pair(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 int launch_batch_wald_ref(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
+int launch_batch_ks_soa(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
+int launch_batch_wald_soa(const DevScene& sc, const BatchLaunch& bl, hipStream_t s);
 
 }  // namespace mh

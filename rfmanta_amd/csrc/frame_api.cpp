@@ -49,7 +49,7 @@ struct mh_frame {
 namespace mh_internal {
 
 int check_launch_config(const mh_launch_config& c) {
-  if (c.layout != MH_LAYOUT_PAIR && c.layout != MH_LAYOUT_REF)
+  if (c.layout != MH_LAYOUT_PAIR && c.layout != MH_LAYOUT_REF && c.layout != MH_LAYOUT_SOA)
     return fail(MH_E_INVALID, "layout %d", c.layout);
   if (c.stack_lds != 8 && c.stack_lds != 16 && c.stack_lds != 32)
     return fail(MH_E_INVALID, "stack_lds %d not in {8,16,32}", c.stack_lds);
@@ -143,12 +143,19 @@ bool spill_ok(const mh_frame* f) {
   return (size_t)need * (size_t)f->spill_stride * 8 <= f->spill.bytes;
 }
 
+// The SoA copy of the scene's PairNodes, current on `stream`, for SOA-layout launches.
+int soa_ready(mh_frame* f, void* stream) {
+  if (f->cfg.layout != MH_LAYOUT_SOA) return MH_OK;
+  return mh_internal::ensure_soa(const_cast<mh_scene*>(f->scene), stream);
+}
+
 int launch(mh_frame* f, void* stream) {
   if (f->total == 0) return MH_OK;
   if (!spill_ok(f)) {
     int rc = prepare(f, f->cfg);
     if (rc) return rc;
   }
+  if (int rc = soa_ready(f, stream)) return rc;
   mh::BatchLaunch bl{};
   bl.table = f->table.as<const mh::BatchTable>();
   bl.nb = f->nb;
@@ -235,7 +242,7 @@ int time_frame(mh_frame* f, int iters, bool graph, void* stream, float* ms) {
   hipEvent_t a, b;
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
-  int rc = MH_OK;
+  int rc = graph ? soa_ready(f, stream) : MH_OK;  // replays read the copy as it stands
   hipError_t e = hipEventRecord(a, st);
   for (int i = 0; i < iters && rc == MH_OK && e == hipSuccess; ++i) {
     if (graph)
@@ -337,6 +344,8 @@ int mh_frame_capture(mh_frame* f, void* stream) {
   int rc = ensure_init();
   if (rc) return rc;
   f->drop_graph();
+  // the SoA copy is brought up to date outside the graph (before each replay as well)
+  if ((rc = soa_ready(f, stream))) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   rc = launch(f, stream);
@@ -357,6 +366,7 @@ int mh_frame_replay(mh_frame* f, void* stream) {
   if (!f) return fail(MH_E_INVALID, "null frame");
   int rc = graph_current(f);
   if (rc) return rc;
+  if ((rc = soa_ready(f, stream))) return rc;
   HIP_OK(hipGraphLaunch(f->exec, reinterpret_cast<hipStream_t>(stream)));
   return MH_OK;
 }
@@ -426,9 +436,9 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
       t.xcd_remap = x;
       if ((rc = trial(t))) return rc;
     }
-  // Stage 3: the reference's 32-byte node layout and the deepest LDS ring.
+  // Stage 3: the SoA and the reference's 32-byte node layouts and the deepest LDS ring.
   base = best;
-  for (int l : {MH_LAYOUT_PAIR, MH_LAYOUT_REF})
+  for (int l : {MH_LAYOUT_PAIR, MH_LAYOUT_SOA, MH_LAYOUT_REF})
     for (int s : {8, 16, 32}) {
       mh_launch_config t = base;
       t.layout = l;

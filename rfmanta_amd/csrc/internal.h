@@ -35,6 +35,10 @@ int trace_fixed_spill(const mh_scene* s, const mh_rays* rays, const mh_hits* hit
                       uint32_t flags, const mh_trace_opts* opts, void* stream, void* spill,
                       size_t spill_bytes);
 
+// Brings the scene's SoA copy of its PairNodes up to date on `stream` (MH_LAYOUT_SOA
+// launches); allocates it on first use.
+int ensure_soa(mh_scene* s, void* stream);
+
 // Validates a batch launch configuration (frame_api.cpp); MH_OK or an error code.
 int check_launch_config(const mh_launch_config& c);
 
@@ -107,6 +111,11 @@ struct mh_scene {
   DevBuf costs, ssize, parent, obox, ocen, anc0, dep0, anc1, dep1, hist, counters;
   DevBuf logtab;  // RTSAH table logarithm (device build)
   DevBuf normals, normal_idx, texcoords, tex_idx;  // shading attributes (mh_shade)
+  // The PairNodes as four arrays of their 16-byte fields (MH_LAYOUT_SOA launches):
+  // allocated on first use and copied from `pairs` again when pairs_gen (bumped by every
+  // call that writes the pair records) has moved past soa_gen.
+  DevBuf pairs_soa;
+  uint64_t pairs_gen = 0, soa_gen = ~0ull;
   std::vector<int64_t> level_off;  // inner records of depth d: levels[level_off[d], level_off[d+1])
 
   std::mutex spill_mu;  // guards the per-stream spill areas
@@ -126,7 +135,7 @@ struct mh_scene {
            d_base.bytes + d_idx.bytes + leaves.bytes + levels.bytes + costs.bytes + ssize.bytes +
            parent.bytes + obox.bytes + ocen.bytes + anc0.bytes + dep0.bytes + anc1.bytes +
            dep1.bytes + hist.bytes + counters.bytes + logtab.bytes + normals.bytes +
-           normal_idx.bytes + texcoords.bytes + tex_idx.bytes;
+           normal_idx.bytes + texcoords.bytes + tex_idx.bytes + pairs_soa.bytes;
   }
   // Traversal stack entries a single ray can hold at most (one push per level).
   uint32_t stack_bound(int /*layout*/) const {
@@ -143,6 +152,7 @@ struct mh_scene {
     d.prim_kind = prim_kind;
     d.n_pairs = (uint32_t)(pairs.bytes / sizeof(mh::PairNode));
     d.n_slots = (uint32_t)(tris.bytes / (12 * sizeof(float)));
+    d.pairs_soa = pairs_soa.p;
     return d;
   }
 };

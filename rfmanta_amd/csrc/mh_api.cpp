@@ -241,6 +241,7 @@ int upload(mh_scene* s, bool rtsah = false) {
   if ((rc = s->costs.reserve(nrec * sizeof(float)))) return rc;
   if ((rc = s->ssize.reserve(nrec * sizeof(uint32_t)))) return rc;
   HIP_OK(hipMemcpy(s->refs.p, nodes.data(), nrec * sizeof(mh::RefNode), hipMemcpyHostToDevice));
+  ++s->pairs_gen;
   HIP_OK(hipMemcpy(s->pairs.p, pairs.data(), n_pairs * sizeof(mh::PairNode),
                    hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s->ids.p, s->bvh.ids.data(), (size_t)T * 4, hipMemcpyHostToDevice));
@@ -353,6 +354,23 @@ int trace_impl(const mh_scene* cs, const mh_rays* rays, const mh_hits* hits, int
 }
 
 }  // namespace
+
+int mh_internal::ensure_soa(mh_scene* s, void* stream) {
+  const size_t n = s->pairs.bytes / sizeof(mh::PairNode);
+  if (n == 0) return fail(MH_E_STATE, "scene has no pair records");
+  if (s->pairs_soa.bytes < n * sizeof(mh::PairNode)) {
+    if (int rc = s->pairs_soa.reserve(n * sizeof(mh::PairNode))) return rc;
+    s->soa_gen = ~0ull;
+  }
+  if (s->soa_gen == s->pairs_gen) return MH_OK;
+  // field k (16 bytes at offset 16k of every 64-byte pair) into array k
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  for (size_t k = 0; k < 4; ++k)
+    HIP_OK(hipMemcpy2DAsync(s->pairs_soa.as<char>() + k * n * 16, 16, s->pairs.as<char>() + 16 * k,
+                            sizeof(mh::PairNode), 16, n, hipMemcpyDeviceToDevice, st));
+  s->soa_gen = s->pairs_gen;
+  return MH_OK;
+}
 
 int mh_internal::trace_fixed_spill(const mh_scene* s, const mh_rays* rays, const mh_hits* hits,
                                    int64_t n, uint32_t flags, const mh_trace_opts* opts,
@@ -589,6 +607,7 @@ int build_device(mh_scene* s, int rtsah) {
   f.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
   f.rtsah = rtsah;
   const int n_levels = (int)level_off.size() - 1;
+  ++s->pairs_gen;
   HIP_OK((hipError_t)mh::build_launch_ranks(f, level_off.data(), n_levels, n_leaves, st));
   HIP_OK(hipMemcpyAsync(cnt, &n_final, 4, hipMemcpyHostToDevice, st));
   mh::DevUpdate u = d.u;
@@ -872,6 +891,7 @@ int mh_scene_refit(mh_scene* s, void* stream) {
   r.n_leaves = (int32_t)s->n_leaves;
   r.n_slots = (int32_t)s->nt;
   r.prim_kind = s->prim_kind;
+  ++s->pairs_gen;
   HIP_OK((hipError_t)mh::launch_refit(r, s->level_off.data(), (int)s->level_off.size() - 1,
                                       stream));
   return MH_OK;
@@ -918,6 +938,7 @@ int mh_scene_update(mh_scene* s, void* stream) {
   // largeSubtreeSize = nodes.size() / (20 * numProcs) (DynBVH.cc:1776)
   u.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
   HIP_OK((hipError_t)mh::launch_obj_bounds(u, st));
+  ++s->pairs_gen;
   HIP_OK((hipError_t)mh::launch_update_tree(u, s->leaves.as<int32_t>(), s->n_leaves,
                                             s->levels.as<int32_t>(), s->level_off.data(),
                                             (int)s->level_off.size() - 1, st));

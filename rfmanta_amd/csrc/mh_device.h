@@ -52,6 +52,7 @@ struct DevScene {
   int32_t root_ref;   // pair-encoded reference of record 0
   int32_t prim_kind;  // 0 KS, 1 Wald
   uint32_t n_pairs;   // PairNodes allocated (the bound of the pair table's descriptor)
+  const void* pairs_soa;  // kLayoutSoa: 16-byte field k of pair p at [k * n_pairs + p], or null
   uint32_t n_slots;   // triangle records allocated (the bound of the triangle table's)
 };
 
@@ -80,6 +81,7 @@ struct LaunchCfg {
 // also held by PairNode (r - 1) / 2, side (r - 1) & 1.
 constexpr int kLayoutPair = 0;  // 64-byte PairNodes, both child boxes per fetch
 constexpr int kLayoutRef = 1;   // 32-byte reference records, own box tested at visit
+constexpr int kLayoutSoa = 2;   // the PairNodes as four arrays of their 16-byte fields
 
 // One ray batch of a persistent launch: rays [start, start + count) of the launch's
 // global numbering are rays [0, count) of this batch.
@@ -113,7 +115,7 @@ struct BatchLaunch {
   int blocks;
   int block;      // 64, 128 or 256 lanes
   int stack_lds;  // 8, 16 or 32
-  int layout;     // kLayoutPair or kLayoutRef
+  int layout;     // kLayoutPair, kLayoutRef or kLayoutSoa
   int refill;     // idle lanes that trigger a refill, 1..63; 64 = whole-wave chunks
   int chunk;      // rays per counter grab (multiple of 64); 0 = static stride, no counter
   int xcd_remap;  // contiguous ray shares per XCD

@@ -351,6 +351,8 @@ int mh_server::queue_scene(int k, const std::vector<std::shared_ptr<Request>>& w
       HIP_OK(hipStreamSynchronize(P.stream));
       if (int rc = P.spill.reserve(spill_bytes)) return rc;
     }
+    if (launch.layout == MH_LAYOUT_SOA)
+      if (int rc = mh_internal::ensure_soa(const_cast<mh_scene*>(scenes[k]), P.stream)) return rc;
     mh::BatchLaunch bl{};
     bl.table = P.table[slot].as<const mh::BatchTable>();
     bl.nb = nb;

@@ -87,7 +87,31 @@ __device__ __forceinline__ bool leaf(const DevScene& sc, int first, int cnt, con
   return false;
 }
 
-template <bool ANY, int PRIM, int S, int BLOCK, bool STATS, bool FAST>
+// The four 16-byte words of PairNode `cur`: one record of the pair table (kLayoutPair), or
+// one entry of each of the four field arrays (kLayoutSoa).
+template <int LAYOUT>
+__device__ __forceinline__ void fetch_pair(const DevScene& sc, int cur, float4& A, float4& B,
+                                           float4& C, int4& M) {
+  float4 Mf;
+  if (LAYOUT == kLayoutSoa) {
+    const unsigned n = sc.n_pairs;
+    const MhBufRsrc t = record_table(sc.pairs_soa, 16, 4 * n);
+    A = fetch_record(t, cur, 0);
+    B = fetch_record(t, (int)(cur + n), 0);
+    C = fetch_record(t, (int)(cur + 2 * n), 0);
+    Mf = fetch_record(t, (int)(cur + 3 * n), 0);
+  } else {
+    const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
+    A = fetch_record(pt, cur, 0);
+    B = fetch_record(pt, cur, 16);
+    C = fetch_record(pt, cur, 32);
+    Mf = fetch_record(pt, cur, 48);
+  }
+  M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
+                __float_as_int(Mf.w));
+}
+
+template <bool ANY, int PRIM, int S, int BLOCK, bool STATS, bool FAST, int LAYOUT = kLayoutPair>
 __device__ __forceinline__ void trace_pair(const DevScene& sc, const RefNode& root, const Ray& r,
                                            RayHit& h, LaneStack<S, BLOCK>& st,
                                            unsigned long long& nbox, unsigned long long& ntri) {
@@ -99,11 +123,9 @@ __device__ __forceinline__ void trace_pair(const DevScene& sc, const RefNode& ro
   st.reset();
   while (go) {
     if (cur >= 0) {
-      const MhBufRsrc pt = record_table(sc.pairs, sizeof(PairNode), sc.n_pairs);
-      const float4 A = fetch_record(pt, cur, 0), B = fetch_record(pt, cur, 16),
-                   C = fetch_record(pt, cur, 32), Mf = fetch_record(pt, cur, 48);
-      const int4 M = make_int4(__float_as_int(Mf.x), __float_as_int(Mf.y), __float_as_int(Mf.z),
-                               __float_as_int(Mf.w));
+      float4 A, B, C;
+      int4 M;
+      fetch_pair<LAYOUT>(sc, cur, A, B, C, M);
       float tn0, tn1;
       const bool h0 = slab_v<FAST>(A.x, A.y, A.z, A.w, B.x, B.y, r, h.minT, tn0);
       const bool h1 = slab_v<FAST>(B.z, B.w, C.x, C.y, C.z, C.w, r, h.minT, tn1);
@@ -181,15 +203,15 @@ __device__ __forceinline__ void trace_store(const DevScene& sc, const RefNode& r
   RayHit h{rays.tmax[i], -1, 0.f, 0.f};
   // The min/max slab test when every ray of the wave allows it (wave-uniform choice).
   if (__ballot(!regular_ray(r, h.minT)) == 0ull) {
-    if (LAYOUT == kLayoutPair)
-      trace_pair<ANY, PRIM, S, BLOCK, STATS, true>(sc, root, r, h, st, nbox, ntri);
-    else
+    if (LAYOUT == kLayoutRef)
       trace_ref<ANY, PRIM, S, BLOCK, STATS, true>(sc, r, h, st, nbox, ntri);
-  } else {
-    if (LAYOUT == kLayoutPair)
-      trace_pair<ANY, PRIM, S, BLOCK, STATS, false>(sc, root, r, h, st, nbox, ntri);
     else
+      trace_pair<ANY, PRIM, S, BLOCK, STATS, true, LAYOUT>(sc, root, r, h, st, nbox, ntri);
+  } else {
+    if (LAYOUT == kLayoutRef)
       trace_ref<ANY, PRIM, S, BLOCK, STATS, false>(sc, r, h, st, nbox, ntri);
+    else
+      trace_pair<ANY, PRIM, S, BLOCK, STATS, false, LAYOUT>(sc, root, r, h, st, nbox, ntri);
   }
   const bool hit = h.best >= 0;
   store_hit(hits, i, hit ? sc.obj_ids[h.best] : -1, (ANY && hit) ? kEps : h.minT, h.u, h.v, hit);

@@ -55,6 +55,11 @@ def _device(batches):
     {"layout": 0, "stack_lds": 16, "block": 64, "blocks_per_cu": 8, "refill": 64, "chunk": 0,
      "xcd_remap": 1},
     {"layout": 1, "stack_lds": 8, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 0},
+    # the PairNodes as four field arrays: static stride, counter chunks, per-lane refill
+    {"layout": 2, "stack_lds": 8, "block": 256, "blocks_per_cu": 4, "refill": 64, "chunk": 0},
+    {"layout": 2, "stack_lds": 16, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 256},
+    {"layout": 2, "stack_lds": 8, "block": 256, "blocks_per_cu": 5, "refill": 32, "chunk": 128,
+     "xcd_remap": 1},
 ])
 def test_frame_matches_oracle(oracle, small, cfg):
     v, i, batches = small
@@ -217,3 +222,48 @@ def test_graph_refused_after_rebuild(small):
             f.close()
         finally:
             api.stream_destroy(stream)
+
+
+def test_soa_layout_follows_refits(oracle):
+    """The SOA layout reads the scene's copy of its PairNodes: after a refit moves the boxes,
+    a launch and a graph replay both see the new ones (results equal the oracle on the
+    refitted tree, frame after frame)."""
+    v, i = scenes.heightfield(120, 60, seed=4)
+    nodes, ids, _ = oracle.build_bvh(v, i)
+    rays = scenes.pinhole_rays((0.0, 0.9, -1.1), (0, 0, 0), (0, 1, 0), 60, 60, 96, 64)
+    cfg = {"layout": 2, "stack_lds": 8, "block": 256, "blocks_per_cu": 2, "refill": 64,
+           "chunk": 0}
+    stream = api.stream_create()
+    with api.Scene(v, i) as s:
+        s.load_dynbvh(ids, nodes)
+        dr = api.DeviceRays.upload(rays)
+        dh = api.DeviceHits.alloc(dr.n)
+        f = api.Frame(s, [(dr, dh, False)])
+        f.set_config(cfg)
+        f.launch()
+        api.sync()
+        f.capture(stream)
+        for frame in range(3):
+            vd = v.copy()
+            vd[:, 1] += np.float32(0.05) * np.sin(np.float32(3.0) * v[:, 0] + np.float32(frame))
+            s.set_vertices(vd)
+            s.refit()
+            api.sync()
+            ref = nodes.copy()
+            oracle.refit(vd, i, ref, ids)
+            o = oracle.trace(vd, i, ref, ids, rays, packet_size=1)
+            for replay in (False, True):
+                dh.arrays["prim"].upload(np.full(dr.n, 777, np.int32))
+                api.sync()
+                if replay:
+                    f.replay(stream)
+                    api.sync(stream)
+                else:
+                    f.launch()
+                    api.sync()
+                g = dh.download()
+                assert np.array_equal(g["prim"], o["prim"]), (frame, replay)
+                assert np.array_equal(_bits(g["t"]), _bits(o["t"])), (frame, replay)
+            assert (o["prim"] >= 0).sum() > 1000
+        f.close()
+    api.stream_destroy(stream)
