@@ -53,7 +53,7 @@ C3_SLO_MS = 100.0     # p99 objective for the sustained-load figure
 # min and a max for the interval, a max and a min into [tmin, tmax]: 8 x 3, plus the final
 # comparison. Kensler-Shirley (KenslerShirleyTriangle.cc:336-377, edges and normal
 # precomputed): det 5, reciprocal 1, p0 - o 3, t 6, its range 3, the cross product 9, u 6
-# and its sign 1, v 6 and its checks 4, the closing interval test 2.
+# and its sign 1, v 6 and its checks 3 (u + v, > 1, v < 0), the closing interval test 2.
 BOX_OPS = 25
 TRI_OPS = 45
 # VALU peaks of MI355X (256 CUs x 4 SIMDs of 16 lanes, 2.4 GHz peak engine clock): one
@@ -643,9 +643,13 @@ def bench_c4(api, scenes, args) -> dict:
     srv.close()
     sc.close()
     total_s = sum(frame_ms) * 1e-3
-    # Refit traffic per frame (leaf boxes from indexed vertices, node boxes written, inner
-    # nodes reading two child boxes) plus the displacement pass (read base, write verts).
-    refit_bytes = nt * 48 + st["nodes"] * 32 + st["inner"] * 48 + nv * 24
+    # Algorithmic bytes of the timed refit (mh_scene_refit): per triangle its id, indices and
+    # vertices read (4 + 12 + 36) and its 48-byte record written; per leaf its list entry and
+    # record read (4 + 32) and its box stored in the record and in the PairNode (2 x 24);
+    # per inner record its list entry, child index and two child boxes read (4 + 4 + 48) and
+    # its box stored twice (2 x 24).
+    n_leaf = st["nodes"] - st["inner"]
+    refit_bytes = nt * (52 + 48) + n_leaf * (36 + 48) + st["inner"] * (56 + 48)
     res = {"frames": args.c4_frames, "tris": nt, "rays_per_frame": rays // args.c4_frames,
            "host_full_sah_build_s": round(host_s, 2),
            "device_approx_build_ms": round(min(approx_ms), 2),
@@ -655,7 +659,9 @@ def bench_c4(api, scenes, args) -> dict:
            "displace_update_ms_mean": round(float(np.mean(update_ms)), 3),
            "displace_update_ms_p99": round(float(np.percentile(update_ms, 99)), 3),
            "refit_ms_mean": round(float(np.mean(refit_ms)), 3),
+           "refit_bytes": int(refit_bytes),
            "refit_gbs": round(refit_bytes / (np.mean(refit_ms) * 1e-3) / 1e9, 1),
+           "refit_hbm_frac": round(refit_bytes / (np.mean(refit_ms) * 1e-3) / 8e12, 3),
            "nodes_before": st["nodes"], "nodes_after": st_end["nodes"],
            "depth_before": st["depth"], "depth_after": st_end["depth"],
            "fps": round(args.c4_frames / total_s, 2), "mrays": round(rays / total_s / 1e6, 1)}

@@ -125,9 +125,11 @@ struct mh_scene {
   void* host_stage = nullptr;         // mh_trace_host: pinned host staging
   size_t host_stage_bytes = 0;
   hipStream_t stage_stream = nullptr;  // mh_trace_host: the scene's own stream
+  unsigned* host_ctr = nullptr;        // mh_scene_update: pinned copy of the level counters
   ~mh_scene() {
     for (auto& kv : spill) delete kv.second;
     if (host_stage) (void)hipHostFree(host_stage);
+    if (host_ctr) (void)hipHostFree(host_ctr);
     if (stage_stream) (void)hipStreamDestroy(stage_stream);
   }
   size_t device_bytes() const {

@@ -910,11 +910,15 @@ int mh_scene_update(mh_scene* s, void* stream) {
   const size_t nrec = s->refs.bytes / sizeof(mh::RefNode);
   // Work arrays, allocated at the first update.
   if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->obox.reserve((size_t)T * 24)) ||
-      (rc = s->ocen.reserve((size_t)T * 12)) || (rc = s->anc0.reserve(nrec * 4)) ||
-      (rc = s->dep0.reserve(nrec * 4)) || (rc = s->anc1.reserve(nrec * 4)) ||
-      (rc = s->dep1.reserve(nrec * 4)) || (rc = s->hist.reserve(mh::kDepthHist * 4)) ||
-      (rc = s->counters.reserve(16)))
+      (rc = s->ocen.reserve((size_t)T * 12)) ||
+      (rc = s->hist.reserve(mh::kFrontierWords * 4)) || (rc = s->counters.reserve(16)))
     return rc;
+  if (!s->host_ctr) {
+    void* h = nullptr;
+    if (hipHostMalloc(&h, (mh::kFrontierWords + 2) * 4, hipHostMallocDefault) != hipSuccess)
+      return fail(MH_E_NOMEM, "hipHostMalloc of the update's counter copy");
+    s->host_ctr = static_cast<unsigned*>(h);
+  }
   unsigned* nfree = s->counters.as<unsigned>();
   int* err = reinterpret_cast<int*>(nfree + 1);
   const unsigned init[2] = {s->next_free, 0u};
@@ -950,58 +954,77 @@ int mh_scene_update(mh_scene* s, void* stream) {
   r.n_slots = (int32_t)T;
   r.prim_kind = s->prim_kind;
   HIP_OK((hipError_t)mh::launch_prim_setup(r, st));
-  // Depth lists of the new topology, for the next refit or update. Pointer jumping takes
-  // enough steps for the previous depth with eight levels to spare (an even count); if the
-  // tree grew deeper than that, the pass says so and runs again with all eight steps.
-  int steps = 2;
-  while (steps < 8 && (1u << steps) <= s->depth + 8) steps += 2;
-  // MH_UPDATE_JUMP_STEPS (2, 4, 6 or 8) forces the first pass's step count, so that tests
-  // can reach the re-run with a tree deeper than the first pass covers.
-  if (const char* e = std::getenv("MH_UPDATE_JUMP_STEPS")) {
-    const int v = std::atoi(e);
-    if (v == 2 || v == 4 || v == 6 || v == 8) steps = v;
+  // Level lists, leaf list and PairNodes of the new topology, top down, one pass per depth
+  // (the small depths at the top in one launch). The first batch of passes covers the
+  // previous depth with two levels to spare; a tree
+  // that grew deeper than that gets more passes after the counters are read back.
+  // MH_UPDATE_FRONTIER_LEVELS (1 or more) sets the first batch's size instead, so that
+  // tests can reach the later batches.
+  const int prev_levels = (int)s->level_off.size() - 1;
+  const unsigned cap = (unsigned)std::max<uint32_t>(1u, T);
+  unsigned* fr = s->hist.as<unsigned>();
+  unsigned* hc = s->host_ctr;
+  const bool inner_root = s->n_inner > 0;  // rotations never turn a leaf root inner
+  int d = 0, d_end = 0;
+  if (inner_root) {
+    d_end = std::min(prev_levels + 2, mh::kMaxDepth - 1);
+    if (const char* e = std::getenv("MH_UPDATE_FRONTIER_LEVELS")) {
+      const int v = std::atoi(e);
+      if (v >= 1) d_end = std::min(v, mh::kMaxDepth - 1);
+    }
+    // the depths that were small in the previous frame run in one workgroup
+    mh::TopLevels t;
+    const int n_top = std::max(1, std::min(mh::top_levels(s->level_off.data(), prev_levels, t),
+                                           d_end));
+    HIP_OK((hipError_t)mh::launch_frontier_top(u, fr, s->levels.as<int32_t>(),
+                                               s->leaves.as<int32_t>(), cap, n_top, st));
+    d = n_top;
   }
-  std::vector<unsigned> hist(mh::kDepthHist);
-  unsigned back[2];
   for (;;) {
-    HIP_OK(hipMemsetAsync(s->hist.p, 0, mh::kDepthHist * 4, st));
-    HIP_OK((hipError_t)mh::launch_depths(u, s->anc0.as<int32_t>(), s->dep0.as<int32_t>(),
-                                         s->anc1.as<int32_t>(), s->dep1.as<int32_t>(),
-                                         s->hist.as<unsigned>(), steps, st));
-    HIP_OK(hipMemcpyAsync(hist.data(), s->hist.p, hist.size() * 4, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(back, nfree, 8, hipMemcpyDeviceToHost, st));
+    for (; d < d_end; ++d) {
+      // workgroups for the depth's size in the previous frame (a grid-stride loop covers
+      // any other size)
+      const int64_t prev = d < prev_levels ? s->level_off[d + 1] - s->level_off[d] : 256;
+      const unsigned blocks = (unsigned)std::min<int64_t>(2048, (prev + prev / 4 + 255) / 256);
+      HIP_OK((hipError_t)mh::launch_frontier(u, fr, s->levels.as<int32_t>(),
+                                             s->leaves.as<int32_t>(), cap, d, blocks, st));
+    }
+    if (inner_root)
+      HIP_OK(hipMemcpyAsync(hc, fr, mh::kFrontierWords * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hc + mh::kFrontierWords, nfree, 8, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
-    if (!hist[mh::kMaxDepth + 2] || steps == 8) break;
-    steps = 8;
+    if (!inner_root || hc[d_end] == 0 || d_end >= mh::kMaxDepth - 1) break;
+    d_end = std::min(d_end + 16, mh::kMaxDepth - 1);
   }
   // Either failure leaves a device tree that must not be traced: the scene needs a build.
-  if (back[1]) {
+  if (hc[mh::kFrontierWords + 1]) {
     s->built = s->updatable = false;
     return fail(MH_E_STATE, "leaf splits ran out of node records; build the scene again");
   }
-  const unsigned deepest = hist[mh::kMaxDepth + 1];
-  if (deepest >= (unsigned)mh::kMaxDepth - 1) {
+  s->next_free = hc[mh::kFrontierWords];
+  if (!inner_root) {
+    s->n_leaves = 1;
+    return MH_OK;
+  }
+  unsigned deepest = 0;
+  while ((int)deepest < d_end && hc[deepest]) ++deepest;
+  if (hc[d_end] || deepest >= (unsigned)mh::kMaxDepth - 1) {
     s->built = s->updatable = false;
     return fail(MH_E_UNSUPPORTED, "tree deeper than %d levels after rotations; build the "
                 "scene again", mh::kMaxDepth - 1);
   }
-  s->next_free = back[0];
-  std::vector<unsigned> cursor(mh::kMaxDepth + 1, 0u);
   s->level_off.assign(1, 0);
-  uint32_t n_inner = 0;
-  for (unsigned d = 0; d < deepest; ++d) {  // inner records lie above the deepest depth
-    cursor[d] = (unsigned)s->level_off.back();
-    s->level_off.push_back(s->level_off.back() + hist[d]);
-    n_inner += hist[d];
+  for (unsigned k = 0; k < deepest; ++k) s->level_off.push_back(s->level_off.back() + hc[k]);
+  const uint64_t n_inner = (uint64_t)s->level_off.back(), n_leaves = hc[mh::kMaxDepth];
+  if (hc[mh::kMaxDepth + 1] || n_leaves != n_inner + 1 || n_leaves > cap) {
+    s->built = s->updatable = false;
+    return fail(MH_E_STATE, "the updated records are not a binary tree (%llu inner, %llu "
+                "leaves); build the scene again", (unsigned long long)n_inner,
+                (unsigned long long)n_leaves);
   }
-  cursor[mh::kMaxDepth] = 0;
-  s->n_inner = n_inner;
-  s->n_leaves = hist[mh::kMaxDepth];
+  s->n_inner = (uint32_t)n_inner;
+  s->n_leaves = (uint32_t)n_leaves;
   s->depth = deepest;
-  HIP_OK(hipMemcpyAsync(s->hist.p, cursor.data(), cursor.size() * 4, hipMemcpyHostToDevice, st));
-  HIP_OK((hipError_t)mh::launch_depth_scatter(u, s->dep0.as<int32_t>(), s->hist.as<unsigned>(),
-                                              s->levels.as<int32_t>(), s->leaves.as<int32_t>(),
-                                              st));
   return MH_OK;
 }
 

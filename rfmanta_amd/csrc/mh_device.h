@@ -145,6 +145,24 @@ struct DevRefit {
 // rotate.hip: DynBVH::update with tree rotations and leaf splits.
 constexpr int kMaxDepth = 256;  // deepest tree the level lists hold
 
+// The top depths of a tree that one workgroup runs in one launch, with a barrier between
+// depths (refit and update): depths [0, n), each of at most kTopNodes records, depth d at
+// [off[d], off[d + 1]) of the level list.
+constexpr int kTopLevels = 32;
+constexpr int kTopNodes = 512;
+struct TopLevels {
+  int32_t off[kTopLevels + 1];
+  int32_t n;
+};
+// The top depths of level_off (n_levels + 1 host offsets) that fit: fills t, returns t.n.
+inline int top_levels(const int64_t* level_off, int n_levels, TopLevels& t) {
+  int n = 0;
+  while (n < n_levels && n < kTopLevels && level_off[n + 1] - level_off[n] <= kTopNodes) ++n;
+  for (int d = 0; d <= n; ++d) t.off[d] = (int32_t)level_off[d];
+  t.n = n;
+  return n;
+}
+
 struct DevUpdate {
   RefNode* refs;
   PairNode* pairs;
@@ -164,7 +182,7 @@ struct DevUpdate {
   uint32_t large;        // largeSubtreeSize
 };
 int launch_obj_bounds(const DevUpdate& u, void* stream);
-// Leaf pass, one launch per depth (deepest first), then the PairNodes.
+// Leaf pass, then one launch per depth (deepest first).
 int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
                        const int32_t* levels, const int64_t* level_off, int n_levels,
                        void* stream);
@@ -179,6 +197,17 @@ int launch_depths(const DevUpdate& u, int32_t* anc0, int32_t* dep0, int32_t* anc
                   unsigned* hist, int steps, void* stream);
 int launch_depth_scatter(const DevUpdate& u, const int32_t* dep, unsigned* cursor,
                          int32_t* levels, int32_t* leaves, void* stream);
+// Level lists and PairNodes of an updated tree, top down from an inner root: counters fr
+// (kFrontierWords: inner records per depth, leaves at [kMaxDepth], an overflow flag at
+// [kMaxDepth + 1], list offsets from [kMaxDepth + 2]). launch_frontier_top clears them and
+// runs depths [0, n_top) (1 <= n_top <= kTopLevels) in one workgroup; launch_frontier(d)
+// then lists depth d + 1 (d < kMaxDepth - 1) and the leaf children of depth d, in
+// `blocks` workgroups. Each list holds at most cap entries.
+constexpr int kFrontierWords = 2 * kMaxDepth + 3;
+int launch_frontier_top(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
+                        unsigned cap, int n_top, void* stream);
+int launch_frontier(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
+                    unsigned cap, int d, unsigned blocks, void* stream);
 
 // refit.hip
 int launch_prim_setup(const DevRefit& r, void* stream);  // tris from verts (all slots)

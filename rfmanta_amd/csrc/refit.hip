@@ -6,7 +6,9 @@
 //     from the current vertices and forms the leaf box (Mesh::getBBox +
 //     BBox::extendByBox, Model/Groups/Mesh.cc:228-237, DynBVH.cc:2091-2097);
 //   k_refit_level: one launch per depth, deepest first; each lane unions the two child
-//     boxes of one inner record (DynBVH.cc:2110-2116).
+//     boxes of one inner record (DynBVH.cc:2110-2116);
+//   k_refit_top: the small depths at the top of the tree (at most kTopNodes records each)
+//     in one workgroup, deepest first, with a barrier between depths.
 // Every box is stored in the reference record and in its PairNode. Kernel boundaries on
 // the stream order the levels, so no cross-workgroup synchronisation is
 // needed (agent-scope acquire/release would write back and invalidate the per-XCD L2s).
@@ -88,10 +90,8 @@ __global__ __launch_bounds__(kBlock) void k_refit_leaves(DevRefit r) {
   store_box(r, rec, lo, hi);
 }
 
-__global__ __launch_bounds__(kBlock) void k_refit_level(DevRefit r, int64_t first, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int rec = r.levels[first + i];
+// Inner record rec: the union of its two child boxes.
+__device__ __forceinline__ void refit_node(const DevRefit& r, int rec) {
   const int c = r.refs[rec].child;
   const RefNode a = r.refs[c];
   const RefNode b = r.refs[c + 1];
@@ -102,6 +102,23 @@ __global__ __launch_bounds__(kBlock) void k_refit_level(DevRefit r, int64_t firs
     hi[k] = pick_hi(pick_hi(-FLT_MAX, a.box.hi[k]), b.box.hi[k]);
   }
   store_box(r, rec, lo, hi);
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_level(DevRefit r, int64_t first, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  refit_node(r, r.levels[first + i]);
+}
+
+// The top depths [0, t.n), deepest first, in one workgroup with a barrier between depths
+// (its waves share one compute unit and its L1, so the boxes stored before the barrier
+// are seen by the loads after it).
+__global__ __launch_bounds__(kBlock) void k_refit_top(DevRefit r, TopLevels t) {
+  for (int d = t.n - 1; d >= 0; --d) {
+    for (int i = t.off[d] + (int)threadIdx.x; i < t.off[d + 1]; i += kBlock)
+      refit_node(r, r.levels[i]);
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_displace_sine(float* verts, const float* base,
@@ -130,12 +147,15 @@ int launch_refit(const DevRefit& r, const int64_t* level_off, int n_levels, void
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned blocks = (unsigned)((r.n_leaves + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_refit_leaves, dim3(blocks), dim3(kBlock), 0, s, r);
-  for (int d = n_levels - 1; d >= 0; --d) {
+  TopLevels t;
+  const int top = top_levels(level_off, n_levels, t);
+  for (int d = n_levels - 1; d >= top; --d) {
     const int64_t first = level_off[d], n = level_off[d + 1] - level_off[d];
     if (n <= 0) continue;
     blocks = (unsigned)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_refit_level, dim3(blocks), dim3(kBlock), 0, s, r, first, n);
   }
+  if (top > 0) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kBlock), 0, s, r, t);
   return (int)hipGetLastError();
 }
 

@@ -12,9 +12,11 @@
 //   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030)
 //   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097)
 //   k_upd_level      one launch per depth: union of the child boxes, then rotateNode
-//   k_rebuild_pairs  PairNodes and big leaf counts from the reference records
-//   k_parents, k_jump, k_depth_hist, k_depth_scatter   depth of every record for the
-//                    next frame's level lists (pointer jumping, then bucketing)
+//   k_upd_top        the same for the small depths at the top, in one workgroup
+//   k_frontier       one launch per depth, top down: the next frame's level lists and
+//                    leaf list, and the PairNodes (and big leaf counts) of the new tree
+// (k_rebuild_pairs and the depth pass k_parents, k_jump, k_depth_hist, k_depth_scatter
+// serve the device build, whose records are not yet in tree order.)
 // Arithmetic is the reference's: Real is float, BBox::computeArea is float arithmetic
 // returned as double, min/max are the select forms, -ffp-contract=off. New record pairs
 // are taken with an atomic counter, so their numbering differs from the reference's
@@ -256,11 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, const int32_
   nd.axis = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_upd_level(DevUpdate u, const int32_t* recs,
-                                                      int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int node = recs[i];
+// One inner node of the update: the union of its child boxes, then rotateNode.
+__device__ __forceinline__ void upd_node(const DevUpdate& u, int node) {
   const int c = u.refs[node].child;
   Box b;
   reset(b);
@@ -268,6 +267,23 @@ __global__ __launch_bounds__(kBlock) void k_upd_level(DevUpdate u, const int32_t
   extend(b, rbox(u.refs[c + 1]));
   set_box(u.refs[node], b);
   rotate_node(u, node);
+}
+
+__global__ __launch_bounds__(kBlock) void k_upd_level(DevUpdate u, const int32_t* recs,
+                                                      int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  upd_node(u, recs[i]);
+}
+
+// The top depths [0, t.n), deepest first, in one workgroup: the barrier between depths
+// orders them as separate launches would (the workgroup's waves share one compute unit
+// and its L1, so stores before the barrier are seen by loads after it).
+__global__ __launch_bounds__(kBlock) void k_upd_top(DevUpdate u, const int32_t* recs, TopLevels t) {
+  for (int d = t.n - 1; d >= 0; --d) {
+    for (int i = t.off[d] + (int)threadIdx.x; i < t.off[d + 1]; i += kBlock) upd_node(u, recs[i]);
+    __syncthreads();
+  }
 }
 
 // Records in use: the counter, capped at the record array (a split that found the array
@@ -301,22 +317,162 @@ __device__ __forceinline__ int32_t encode_rec(const DevUpdate& u, const RecWords
   return ~((child << kLeafCountBits) | c);
 }
 
-// The PairNode of every inner record in use, written as four 16-byte stores from the two
-// child records read as four 16-byte loads.
-__global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= used(u.next_free, u.n_records)) return;
-  const RecWords nd = load_rec(u.refs, r);
-  if (rec_children(nd) != 0) return;
-  const int c = rec_child(nd);
-  const RecWords c0 = load_rec(u.refs, c), c1 = load_rec(u.refs, c + 1);
-  float4* P = reinterpret_cast<float4*>(u.pairs + ((c - 1) >> 1));
+// The PairNode of inner record r (loaded as nd) from its two child records c0, c1, as four
+// 16-byte stores.
+__device__ __forceinline__ void store_pair(const DevUpdate& u, int64_t r, const RecWords& nd,
+                                           const RecWords& c0, const RecWords& c1) {
+  float4* P = reinterpret_cast<float4*>(u.pairs + ((rec_child(nd) - 1) >> 1));
   P[0] = make_float4(c0.a.x, c0.a.w, c0.a.y, c0.b.x);  // child 0: lo.x hi.x lo.y hi.y
   P[1] = make_float4(c0.a.z, c0.b.y, c1.a.x, c1.a.w);  // lo.z hi.z | child 1: lo.x hi.x
   P[2] = make_float4(c1.a.y, c1.b.x, c1.a.z, c1.b.y);  // lo.y hi.y lo.z hi.z
   const int word = __float_as_int(nd.b.w);
   reinterpret_cast<int4*>(P)[3] =
       make_int4(encode_rec(u, c0), encode_rec(u, c1), (word & 3) | (((word >> 8) & 1) << 2), (int)r);
+}
+
+// The PairNode of every inner record in use.
+__global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= used(u.next_free, u.n_records)) return;
+  const RecWords nd = load_rec(u.refs, r);
+  if (rec_children(nd) != 0) return;
+  const int c = rec_child(nd);
+  store_pair(u, r, nd, load_rec(u.refs, c), load_rec(u.refs, c + 1));
+}
+
+// Level lists of the updated tree, top down, and its PairNodes. fr holds kFrontierWords
+// counters: fr[d] inner records of depth d (d < kMaxDepth - 1), fr[kMaxDepth] leaves,
+// fr[kMaxDepth + 1] set when a list would pass its capacity, and from fr + kMaxDepth + 2
+// the list offsets: depth d is levels[off[d], off[d] + fr[d]). The root is an inner record.
+//
+// One workgroup's share of depth d, whose records are levels[first, first + n): items
+// i_begin, i_begin + stride, ... Each record writes the PairNode of its two children and
+// appends them, in child order, to depth d + 1 (inner; that list starts at first + n) or
+// to the leaf list. Per loop iteration the workgroup ranks its appends with ballots and
+// takes its ranges with one call of reserve(inner, leaves, inner base, leaf base) by its
+// first lane. Order inside a list is free: the nodes of one depth are independent.
+template <class Reserve>
+__device__ __forceinline__ void frontier_depth(const DevUpdate& u, unsigned* fr, int32_t* levels,
+                                               int32_t* leaves, unsigned cap, unsigned first,
+                                               unsigned n, unsigned i_begin, unsigned stride,
+                                               Reserve reserve) {
+  __shared__ unsigned w_in[kBlock / 64], w_lf[kBlock / 64];
+  __shared__ unsigned b_in, b_lf;
+  const unsigned next = first + n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  auto put = [&](int32_t* list, unsigned pos, int v) {
+    if (pos < cap)
+      list[pos] = v;
+    else
+      fr[kMaxDepth + 1] = 1u;
+  };
+  for (unsigned i0 = i_begin; i0 < n; i0 += stride) {  // uniform over the workgroup
+    const unsigned i = i0 + threadIdx.x;
+    const bool live = i < n && first + i < cap;  // past cap only after an overflow (flagged)
+    int c = 0;
+    bool in0 = false, in1 = false;
+    if (live) {
+      const int r = levels[first + i];
+      const RecWords nd = load_rec(u.refs, r);
+      c = rec_child(nd);
+      const RecWords c0 = load_rec(u.refs, c), c1 = load_rec(u.refs, c + 1);
+      store_pair(u, r, nd, c0, c1);
+      in0 = rec_children(c0) == 0;
+      in1 = rec_children(c1) == 0;
+    }
+    const int k_in = live ? (int)in0 + (int)in1 : 0, k_lf = live ? 2 - k_in : 0;
+    const unsigned long long i1 = __ballot(k_in >= 1), i2 = __ballot(k_in == 2);
+    const unsigned long long l1 = __ballot(k_lf >= 1), l2 = __ballot(k_lf == 2);
+    if (lane == 0) {
+      w_in[wave] = (unsigned)(__popcll(i1) + __popcll(i2));
+      w_lf[wave] = (unsigned)(__popcll(l1) + __popcll(l2));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned t_in = 0, t_lf = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        const unsigned x = w_in[w], y = w_lf[w];
+        w_in[w] = t_in;
+        w_lf[w] = t_lf;
+        t_in += x;
+        t_lf += y;
+      }
+      unsigned bi = 0, bl = 0;
+      reserve(t_in, t_lf, bi, bl);
+      b_in = bi;
+      b_lf = bl;
+    }
+    __syncthreads();
+    if (live) {
+      unsigned pi = next + b_in + w_in[wave] + (unsigned)(__popcll(i1 & below) + __popcll(i2 & below));
+      unsigned pl = b_lf + w_lf[wave] + (unsigned)(__popcll(l1 & below) + __popcll(l2 & below));
+      if (in0)
+        put(levels, pi++, c);
+      else
+        put(leaves, pl++, c);
+      if (in1)
+        put(levels, pi, c + 1);
+      else
+        put(leaves, pl, c + 1);
+    }
+    __syncthreads();  // w_in, w_lf, b_in, b_lf are rewritten by the next iteration
+  }
+}
+
+// Depths [0, n_top) in one workgroup, from the root: it clears fr, counts in LDS (a
+// barrier ends each depth; the workgroup's waves share one compute unit and its L1, so
+// the list entries stored before it are seen after it) and leaves fr[0..n_top], the
+// offsets up to depth n_top and the leaf count for the launches of the deeper depths.
+__global__ __launch_bounds__(kBlock) void k_frontier_top(DevUpdate u, unsigned* fr, int32_t* levels,
+                                                         int32_t* leaves, unsigned cap, int n_top) {
+  __shared__ unsigned s_cnt[kTopLevels + 1], s_off[kTopLevels + 1], s_leaf;
+  for (int i = threadIdx.x; i < kFrontierWords; i += kBlock) fr[i] = 0u;
+  if (threadIdx.x == 0) {
+    s_cnt[0] = 1u;  // depth 0: the root, at levels[0]
+    s_off[0] = 0u;
+    s_leaf = 0u;
+    levels[0] = 0;
+  }
+  __syncthreads();
+  for (int d = 0; d < n_top; ++d) {
+    const unsigned n = s_cnt[d], first = s_off[d];
+    __syncthreads();  // every lane has read the depth's count before lane 0 opens the next
+    if (threadIdx.x == 0) {
+      s_off[d + 1] = first + n;
+      s_cnt[d + 1] = 0u;
+    }
+    __syncthreads();
+    frontier_depth(u, fr, levels, leaves, cap, first, n, 0u, kBlock,
+                   [&](unsigned t_in, unsigned t_lf, unsigned& bi, unsigned& bl) {
+                     bi = s_cnt[d + 1];
+                     s_cnt[d + 1] = bi + t_in;
+                     bl = s_leaf;
+                     s_leaf = bl + t_lf;
+                   });
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    unsigned* off = fr + kMaxDepth + 2;
+    for (int d = 0; d <= n_top; ++d) {
+      fr[d] = s_cnt[d];
+      off[d] = s_off[d];
+    }
+    fr[kMaxDepth] = s_leaf;
+  }
+}
+
+// Depth d (>= the depths of k_frontier_top) over the whole grid, counters in fr.
+__global__ __launch_bounds__(kBlock) void k_frontier(DevUpdate u, unsigned* fr, int32_t* levels,
+                                                     int32_t* leaves, unsigned cap, int d) {
+  unsigned* off = fr + kMaxDepth + 2;
+  const unsigned n = fr[d], first = off[d];
+  if (blockIdx.x == 0 && threadIdx.x == 0) off[d + 1] = first + n;
+  frontier_depth(u, fr, levels, leaves, cap, first, n, blockIdx.x * kBlock, gridDim.x * kBlock,
+                 [&](unsigned t_in, unsigned t_lf, unsigned& bi, unsigned& bl) {
+                   bi = t_in ? atomicAdd(fr + d + 1, t_in) : 0u;
+                   bl = t_lf ? atomicAdd(fr + kMaxDepth, t_lf) : 0u;
+                 });
 }
 
 // anc = parent record (root: -1), dep = 1 (root: 0) for every record in use.
@@ -510,12 +666,30 @@ int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leav
   if (n_leaves > 0)
     hipLaunchKernelGGL(k_upd_leaves, dim3(grid(n_leaves)), dim3(kBlock), 0, s, u, leaves,
                        n_leaves);
-  for (int d = n_levels - 1; d >= 0; --d) {
+  TopLevels t;
+  const int top = top_levels(level_off, n_levels, t);
+  for (int d = n_levels - 1; d >= top; --d) {
     const int64_t first = level_off[d], n = level_off[d + 1] - level_off[d];
     if (n > 0)
       hipLaunchKernelGGL(k_upd_level, dim3(grid(n)), dim3(kBlock), 0, s, u, levels + first, n);
   }
-  hipLaunchKernelGGL(k_rebuild_pairs, dim3(grid(u.n_records)), dim3(kBlock), 0, s, u);
+  if (top > 0) hipLaunchKernelGGL(k_upd_top, dim3(1), dim3(kBlock), 0, s, u, levels, t);
+  return (int)hipGetLastError();
+}
+
+int launch_frontier_top(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
+                        unsigned cap, int n_top, void* stream) {
+  if (n_top < 1 || n_top > kTopLevels || n_top >= kMaxDepth - 1) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_frontier_top, dim3(1), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), u, fr, levels, leaves, cap, n_top);
+  return (int)hipGetLastError();
+}
+
+int launch_frontier(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
+                    unsigned cap, int d, unsigned blocks, void* stream) {
+  if (d < 0 || d >= kMaxDepth - 1) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_frontier, dim3(blocks ? blocks : 1), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), u, fr, levels, leaves, cap, d);
   return (int)hipGetLastError();
 }
 

@@ -89,13 +89,13 @@ def test_update_large_tree_matches_oracle(oracle):
             assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32)), f
 
 
-@pytest.mark.parametrize("mesh", ["heightfield", "soup", "heightfield_short_jump"])
+@pytest.mark.parametrize("mesh", ["heightfield", "soup", "heightfield_short_frontier"])
 def test_update_matches_oracle(oracle, mesh, monkeypatch):
-    if mesh == "heightfield_short_jump":
-        # The depth pass starts with two pointer-jumping steps (depth 4): every frame's
-        # tree is deeper, so the pass flags records short of the root and runs again with
-        # eight; the next frame's level lists come from that second run.
-        monkeypatch.setenv("MH_UPDATE_JUMP_STEPS", "2")
+    if mesh == "heightfield_short_frontier":
+        # The level passes start with one depth: every frame's tree is deeper, so the
+        # update reads the counters back and runs the remaining depths in a second batch;
+        # the next frame's level lists come from both batches.
+        monkeypatch.setenv("MH_UPDATE_FRONTIER_LEVELS", "1")
         mesh = "heightfield"
     if mesh == "heightfield":
         v, i = scenes.heightfield(120, 60, seed=2)
