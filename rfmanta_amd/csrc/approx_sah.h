@@ -65,17 +65,33 @@ __device__ __forceinline__ void set_box(RefNode& n, const Box& b) {
     n.box.hi[k] = b.hi[k];
   }
 }
+// Object box: from u.obox when the caller filled it (k_obj_bounds), else from the
+// triangle's vertices with the same arithmetic (Mesh::getBBox, Model/Groups/Mesh.cc:228-237).
 __device__ __forceinline__ Box obj_box(const DevUpdate& u, int obj) {
   Box b;
-  const float* p = u.obox + 6 * (size_t)obj;
-  for (int k = 0; k < 3; ++k) {
-    b.lo[k] = p[k];
-    b.hi[k] = p[3 + k];
+  if (u.obox) {
+    const float* p = u.obox + 6 * (size_t)obj;
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = p[k];
+      b.hi[k] = p[3 + k];
+    }
+    return b;
   }
+  const uint32_t* t = u.idx + 3 * (size_t)obj;
+  const float* p0 = u.verts + 3 * (size_t)t[0];
+  for (int k = 0; k < 3; ++k) b.lo[k] = b.hi[k] = p0[k];
+  extend_pt(b, u.verts + 3 * (size_t)t[1]);
+  extend_pt(b, u.verts + 3 * (size_t)t[2]);
   return b;
 }
+// BBox::center = Interpolate(min, max, 0.5f) (BBox.h:63-65, MiscMath.h:136-139)
+__device__ __forceinline__ float box_cen(const Box& b, int axis) {
+  return b.hi[axis] * 0.5f + b.lo[axis] * (1 - 0.5f);
+}
+// Object centroid: from u.ocen when filled, else from the object box.
 __device__ __forceinline__ float obj_cen(const DevUpdate& u, int obj, int axis) {
-  return u.ocen[3 * (size_t)obj + axis];
+  if (u.ocen) return u.ocen[3 * (size_t)obj + axis];
+  return box_cen(obj_box(u, obj), axis);
 }
 // Vector::indexOfMaxComponent (Core/Geometry/Vector.h:382-393)
 __device__ __forceinline__ int index_of_max(float a, float b, float c) {
@@ -92,18 +108,27 @@ struct Partition {
   int axis, split;
 };
 
+// partition2Objs's decision (DynBVH.cc:2277-2290): whether two objects with boxes left,
+// right under a node with box node_box are split.
+__device__ inline bool pair_splits(const Box& node_box, const Box& left, const Box& right) {
+  const float nodeArea = area(node_box);
+  const float leftArea = area(left);
+  const float rightArea = area(right);
+  const float leafCost = kCIsec * 2;
+  const float newCost = kCTrav + (leftArea * kCIsec + rightArea * kCIsec) / nodeArea;
+  return newCost < leafCost;
+}
+__device__ inline bool pair_splits(const DevUpdate& u, const Box& node_box, int b) {
+  return pair_splits(node_box, obj_box(u, u.ids[b]), obj_box(u, u.ids[b + 1]));
+}
+
 // partition2Objs (DynBVH.cc:2277-2308)
 __device__ inline Partition partition2(const DevUpdate& u, int node, int b) {
   Partition p;
   reset(p.l);
   reset(p.r);
   int32_t* ids = u.ids;
-  const float nodeArea = area(rbox(u.refs[node]));
-  const float leftArea = area(obj_box(u, ids[b]));
-  const float rightArea = area(obj_box(u, ids[b + 1]));
-  const float leafCost = kCIsec * 2;
-  const float newCost = kCTrav + (leftArea * kCIsec + rightArea * kCIsec) / nodeArea;
-  if (newCost < leafCost) {
+  if (pair_splits(u, rbox(u.refs[node]), b)) {
     const Box nb = rbox(u.refs[node]);
     const int axis = index_of_max(nb.hi[0] - nb.lo[0], nb.hi[1] - nb.lo[1], nb.hi[2] - nb.lo[2]);
     p.axis = axis;
@@ -141,27 +166,30 @@ __device__ inline int partition_objects(const DevUpdate& u, int b, int e, int ax
   }
 }
 
-// partitionApproxSAH (DynBVH.cc:2125-2275); LONGEST_AXIS is defined (DynBVH.cc:39), so
-// only the centroid box's longest axis is binned.
-__device__ inline Partition partition_approx(const DevUpdate& u, int node, int b, int e) {
-  Partition p;
-  reset(p.l);
-  reset(p.r);
+// The binned SAH choice of partitionApproxSAH (DynBVH.cc:2125-2246) for n >= 3 objects
+// [b, e) under a node with box node_box: the split axis (-1: the node stays a leaf) and
+// position. LONGEST_AXIS is defined (DynBVH.cc:39), so only the centroid box's longest
+// axis is binned. Reads only.
+struct SplitChoice {
+  int axis;
+  float pos;
+};
+__device__ inline SplitChoice approx_choice(const DevUpdate& u, const Box& node_box, int b, int e) {
   const int n = e - b;
-  if (n == 1) {
-    p.axis = -1;
-    p.split = -1;
-    return p;
-  }
-  if (n == 2) return partition2(u, node, b);
   const int32_t* ids = u.ids;
   Box cb;
   reset(cb);
   for (int i = b; i < e; ++i) {
-    const float c[3] = {obj_cen(u, ids[i], 0), obj_cen(u, ids[i], 1), obj_cen(u, ids[i], 2)};
+    float c[3];
+    if (u.ocen) {
+      for (int k = 0; k < 3; ++k) c[k] = u.ocen[3 * (size_t)ids[i] + k];
+    } else {
+      const Box ob = obj_box(u, ids[i]);
+      for (int k = 0; k < 3; ++k) c[k] = box_cen(ob, k);
+    }
     extend_pt(cb, c);
   }
-  const float inv_parent_area = (float)(1.0 / (double)area(rbox(u.refs[node])));
+  const float inv_parent_area = (float)(1.0 / (double)area(node_box));
   float best_cost = n * kCIsec;
   int best_axis = -1;
   float best_pos = FLT_MAX;
@@ -175,13 +203,15 @@ __device__ inline Partition partition_approx(const DevUpdate& u, int node, int b
     counts[i] = 0;
   }
   for (int i = b; i < e; ++i) {
-    const float f = (obj_cen(u, ids[i], axis) - cb.lo[axis]) * scale;
+    const Box ob = obj_box(u, ids[i]);
+    const float cen = u.ocen ? u.ocen[3 * (size_t)ids[i] + axis] : box_cen(ob, axis);
+    const float f = (cen - cb.lo[axis]) * scale;
     // int(f) as the reference's x86 build converts it (cvttss2si): NaN and values outside
     // the int range give INT_MIN, i.e. bin 0 after the clamp.
     int which = (f >= -2147483648.f && f < 2147483648.f) ? (int)f : INT_MIN;
     which = which < 0 ? 0 : (which > kSamples - 1 ? kSamples - 1 : which);
     counts[which]++;
-    extend(bins[which], obj_box(u, ids[i]));
+    extend(bins[which], ob);
   }
   float left_areas[kSamples], right_areas[kSamples];
   int left_counts[kSamples], right_counts[kSamples];
@@ -210,6 +240,34 @@ __device__ inline Partition partition_approx(const DevUpdate& u, int node, int b
       best_pos = cb.lo[axis] + width * (static_cast<float>(i + 1) / kSamples);
     }
   }
+  return SplitChoice{best_axis, best_pos};
+}
+
+// Whether the approximate SAH build splits objects [b, e) under a node with box node_box
+// (partitionApproxSAH's decision, the ids not moved).
+__device__ inline bool approx_splits(const DevUpdate& u, const Box& node_box, int b, int e) {
+  const int n = e - b;
+  if (n <= 1) return false;
+  if (n == 2) return pair_splits(u, node_box, b);
+  return approx_choice(u, node_box, b, e).axis != -1;
+}
+
+// partitionApproxSAH (DynBVH.cc:2125-2275)
+__device__ inline Partition partition_approx(const DevUpdate& u, int node, int b, int e) {
+  Partition p;
+  reset(p.l);
+  reset(p.r);
+  const int n = e - b;
+  if (n == 1) {
+    p.axis = -1;
+    p.split = -1;
+    return p;
+  }
+  if (n == 2) return partition2(u, node, b);
+  const int32_t* ids = u.ids;
+  const SplitChoice ch = approx_choice(u, rbox(u.refs[node]), b, e);
+  const int best_axis = ch.axis;
+  const float best_pos = ch.pos;
   p.axis = best_axis;
   if (p.axis != -1) {
     int middle = partition_objects(u, b, e, p.axis, best_pos);
@@ -321,11 +379,22 @@ __device__ inline void build_loop(const DevUpdate& u, Pairs& pairs, int start, i
   }
 }
 
+// A leaf bit of the rotating update, outside the reference's two (bit 0 isLeftCheaper,
+// bit 1 isLargeSubtree): set by the leaf pass when the approximate SAH keeps the leaf
+// whole, so that its parent's rotation need not read the objects again, and cleared there.
+// Every marked leaf is still its parent's child when the parent rotates (rotations below
+// do not reach a sibling leaf), so no mark outlives the frame.
+constexpr uint8_t kKeepLeaf = 0x80;
+
 // build(leaf, begin, end, useApproximateBuild = true, nodeAlreadyExists = true), as
 // rotateNode calls it on a leaf child (DynBVH.cc:3057-3069): the leaf stays unless the
 // approximate SAH splits it.
 __device__ inline void split_leaf(const DevUpdate& u, int leaf) {
   RefNode* refs = u.refs;
+  if (refs[leaf].bits & kKeepLeaf) {  // decided by the leaf pass on the same objects and box
+    refs[leaf].bits &= (uint8_t)~kKeepLeaf;
+    return;
+  }
   const int b0 = refs[leaf].child, e0 = b0 + refs[leaf].children;
   const Partition p0 = partition_approx(u, leaf, b0, e0);
   if (p0.axis == -1) return;

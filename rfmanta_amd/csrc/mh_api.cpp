@@ -909,9 +909,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   const uint32_t T = s->nt;
   const size_t nrec = s->refs.bytes / sizeof(mh::RefNode);
   // Work arrays, allocated at the first update.
-  if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->obox.reserve((size_t)T * 24)) ||
-      (rc = s->ocen.reserve((size_t)T * 12)) ||
-      (rc = s->hist.reserve(mh::kFrontierWords * 4)) || (rc = s->counters.reserve(16)))
+  if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->hist.reserve(mh::kFrontierWords * 4)) ||
+      (rc = s->counters.reserve(16)))
     return rc;
   if (!s->host_ctr) {
     void* h = nullptr;
@@ -931,8 +930,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   u.ssize = s->ssize.as<uint32_t>();
   u.parent = s->parent.as<int32_t>();
   u.ids = s->ids.as<int32_t>();
-  u.obox = s->obox.as<float>();
-  u.ocen = s->ocen.as<float>();
+  u.obox = nullptr;  // object boxes and centroids from the vertices, where they are needed
+  u.ocen = nullptr;
   u.verts = s->d_verts.as<float>();
   u.idx = s->d_idx.as<uint32_t>();
   u.next_free = nfree;
@@ -941,7 +940,6 @@ int mh_scene_update(mh_scene* s, void* stream) {
   u.n_objects = (int32_t)T;
   // largeSubtreeSize = nodes.size() / (20 * numProcs) (DynBVH.cc:1776)
   u.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
-  HIP_OK((hipError_t)mh::launch_obj_bounds(u, st));
   ++s->pairs_gen;
   HIP_OK((hipError_t)mh::launch_update_tree(u, s->leaves.as<int32_t>(), s->n_leaves,
                                             s->levels.as<int32_t>(), s->level_off.data(),

@@ -171,8 +171,8 @@ struct DevUpdate {
   uint32_t* ssize;       // DynBVH::subtree_size, per record
   int32_t* parent;       // parent links of records built by leaf splits
   int32_t* ids;          // object_ids (leaf splits permute ranges)
-  float* obox;           // per object: lo xyz, hi xyz
-  float* ocen;           // per object: centroid xyz
+  float* obox;           // per object: lo xyz, hi xyz (null: formed from verts and idx)
+  float* ocen;           // per object: centroid xyz (null: formed from the object box)
   const float* verts;
   const uint32_t* idx;
   unsigned* next_free;   // records in use (nextFree)

@@ -9,8 +9,10 @@
 // (DynBVH.cc:3051-3372). A node only touches records of its own subtree, so all nodes of
 // one depth (depth taken at the start of the frame) run in parallel, deepest first, one
 // thread per node:
-//   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030)
-//   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097)
+//   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030), for the device
+//                    build; the update forms them from the vertices where it needs them
+//   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097), and whether each leaf stays whole
+//                    when its parent's rotation tries to split it
 //   k_upd_level      one launch per depth: union of the child boxes, then rotateNode
 //   k_upd_top        the same for the small depths at the top, in one workgroup
 //   k_frontier       one launch per depth, top down: the next frame's level lists and
@@ -250,12 +252,26 @@ __global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, const int32_
                                                        int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  RefNode& nd = u.refs[leaves[i]];
+  const int rec = leaves[i];
+  RefNode& nd = u.refs[rec];
+  const int first = nd.child, cnt = nd.children;
   Box b;
   reset(b);
-  for (int k = 0; k < nd.children; ++k) extend(b, obj_box(u, u.ids[nd.child + k]));
+  // The parent's rotation will ask whether this leaf splits: decide it now, from the same
+  // object boxes (a root leaf has no parent to ask, a one-object leaf never splits).
+  bool keep;
+  if (cnt == 2) {  // the common leaf: both boxes once, for the leaf box and partition2Objs
+    const Box o0 = obj_box(u, u.ids[first]), o1 = obj_box(u, u.ids[first + 1]);
+    extend(b, o0);
+    extend(b, o1);
+    keep = !pair_splits(b, o0, o1);
+  } else {
+    for (int k = 0; k < cnt; ++k) extend(b, obj_box(u, u.ids[first + k]));
+    keep = cnt > 2 && !approx_splits(u, b, first, first + cnt);
+  }
   set_box(nd, b);
   nd.axis = 0;
+  if (rec != 0 && keep) nd.bits |= kKeepLeaf;
 }
 
 // One inner node of the update: the union of its child boxes, then rotateNode.
