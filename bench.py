@@ -39,6 +39,7 @@ import numpy as np
 
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
+from rfmanta_amd import scenes as scenes_mod  # noqa: E402  (numpy only; no library load)
 
 METRIC = "Mrays/s closest-hit at 1M tris (coherent/incoherent); p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
@@ -62,13 +63,10 @@ TRI_OPS = 45
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 PROFILE_JSON = "profiles/counters_trace_batch_heightfield1M.json"
-# Launch configuration of the timed C1 frame: the round refill (a lane takes a new ray when
-# 32 of its wave's lanes are idle, rays handed out 256 at a time), 12 workgroups of 128
-# lanes per CU, LDS ring of 8, PAIR nodes; the autotuner's pick on MI355X, 1.5-1.7% ahead
-# of the static stride in repeated timings of this frame. Pinned so that the counter profile
-# of the kernel is taken on the very launch that is timed.
-C1_CONFIG = {"layout": 0, "stack_lds": 8, "block": 128, "blocks_per_cu": 12, "refill": 32,
-             "chunk": 256, "xcd_remap": 0}
+# Launch configuration of the timed C1 frame (rfmanta_amd.scenes.C1_LAUNCH, which the
+# full-size GPU tests check against the oracle): pinned so that the counter profile of the
+# kernel is taken on the very launch that is timed.
+C1_CONFIG = scenes_mod.C1_LAUNCH
 KERNEL_SOURCES = ("batch_kernel.h", "traverse.h", "trace_common.h", "mh_device.h",
                   "batch_ks_pair.hip")
 

@@ -31,9 +31,10 @@ pytestmark = pytest.mark.gpu
 
 from rfmanta_amd import api, scenes  # noqa: E402
 
-# Launch configurations the benchmark's autotuner picks on these workloads.
-C1_FRAME = {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 24, "refill": 64,
-            "chunk": 0, "xcd_remap": 0}
+# Launch configurations the benchmark's autotuner picks on these workloads; C1_FRAME is the
+# one bench.py pins for its timed C1 frame.
+C1_FRAME = scenes.C1_LAUNCH
+C1_STATIC = scenes.C1_LAUNCH_STATIC
 C1_COHERENT = {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 16, "refill": 64,
                "chunk": 0, "xcd_remap": 0}
 C1_SHADOW = {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 24, "refill": 64,
@@ -105,7 +106,8 @@ def _scribble(dh):
     dh.arrays["t"].upload(np.full(dh.n, -1.0, np.float32))
 
 
-@pytest.mark.parametrize("cfg", [C1_FRAME, "autotune"], ids=["bench_pick", "live_pick"])
+@pytest.mark.parametrize("cfg", [C1_FRAME, C1_STATIC, "autotune"],
+                         ids=["bench_pick", "static_stride", "live_pick"])
 def test_c1_frame_matches_oracle(c1, cfg):
     """The benchmark's timed launch: one k_trace_batch over the primary + bounce batches."""
     print(f"c1 frame {cfg}", flush=True)
@@ -127,8 +129,8 @@ def test_c1_frame_matches_oracle(c1, cfg):
     f.close()
 
 
-@pytest.mark.parametrize("name,cfg", [("primary", C1_COHERENT), ("bounce", C1_FRAME),
-                                      ("shadow", C1_SHADOW)])
+@pytest.mark.parametrize("name,cfg", [("primary", C1_COHERENT), ("bounce", C1_STATIC),
+                                      ("bounce", C1_FRAME), ("shadow", C1_SHADOW)])
 def test_c1_single_set_frames_match_oracle(c1, name, cfg):
     print(f"c1 single-set frame {name} {cfg}", flush=True)
     dr, dh, a = c1["dev"][name]
