@@ -100,6 +100,9 @@ def main() -> int:
     ap.add_argument("--c1-config", default=json.dumps(C1_CONFIG),
                     help="JSON launch configuration of the timed C1 frame ('' = the "
                          "autotuner's pick)")
+    ap.add_argument("--server-launch", default="",
+                    help="JSON launch configuration of the C3/C4 request loops ('' = the "
+                         "library's default)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner run reported beside the C1 frame")
     ap.add_argument("--c4-frames", type=int, default=100)
@@ -523,7 +526,8 @@ def bench_c3(api, scenes, scene_hf, args) -> dict:
     soup.build()
     log(f"C3: soup 8M tris generated+built in {time.time() - t0:.1f}s")
     srv = api.Server([scene_hf, soup], hbm_budget=8 << 30, host_budget=2 << 30,
-                     chunk_rays=1 << 21, max_active=64, shadows=True, light=scenes.C3_LIGHT)
+                     chunk_rays=1 << 21, max_active=64, shadows=True, light=scenes.C3_LIGHT,
+                     launch=json.loads(args.server_launch) if args.server_launch else None)
     per_point = 64
     qps_list = (25, 50, 100, 200, 400, 800, 1600, 3200)
     reqs = scenes.c3_requests(per_point * (len(qps_list) + 1))
@@ -613,6 +617,7 @@ def bench_c4(api, scenes, args) -> dict:
     log(f"C4: {nt} tris generated+built in {time.time() - t0:.1f}s (host full SAH "
         f"{host_s:.2f}s, device approximate {approx_ms} ms), {st}")
     srv = api.Server([sc], hbm_budget=2 << 30, host_budget=1 << 30, chunk_rays=1 << 22,
+                     launch=json.loads(args.server_launch) if args.server_launch else None,
                      shadows=True, bounces=True, light=(2.0, 2.0, -2.0))
     cam = api.camera(**scenes.C4_CAMERA)
     refit_ms, frame_ms, rays, hits = [], [], 0, 0

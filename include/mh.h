@@ -394,7 +394,8 @@ typedef struct mh_server_opts {
   int bounces;         /* 1 = one diffuse bounce ray per primary ray (closest-hit), as
                           mh_gen_bounce_rays with seed = camera.seed + 1 [0] */
   float light[3];
-  mh_launch_config launch; /* zero = default configuration */
+  mh_launch_config launch; /* zero = default configuration: PAIR layout, LDS ring 8,
+                              256-lane workgroups, 16 per CU, static stride */
 } mh_server_opts;
 
 typedef struct mh_request {

@@ -616,7 +616,12 @@ int mh_server_create(const mh_scene* const* scenes, int n_scenes, const mh_serve
   if (d.max_active <= 0) d.max_active = 64;
   if (d.max_active > kTableCap / 2) d.max_active = kTableCap / 2;  // two table entries each
   s->opts = d;
-  s->launch = (d.launch.block == 0) ? mh_launch_config{MH_LAYOUT_PAIR, 8, 256, 8, 64, 256, 0}
+  // Default: the static stride, 16 workgroups of 256 lanes per CU. The waves in flight
+  // then cover a contiguous part of each batch, which keeps the records they read in the
+  // caches; counter chunks of 256 rays spread the first chunks of all waves over a whole
+  // 1080p batch, and on an 8M-triangle scene (records larger than the Infinity Cache)
+  // traced its primary rays 2.8 times slower.
+  s->launch = (d.launch.block == 0) ? mh_launch_config{MH_LAYOUT_PAIR, 8, 256, 16, 64, 0, 0}
                                     : d.launch;
   if ((rc = mh_internal::check_launch_config(s->launch))) return rc;
   if ((rc = s->arena_mem.reserve(d.hbm_budget))) return rc;
