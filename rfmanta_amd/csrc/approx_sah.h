@@ -243,15 +243,6 @@ __device__ inline SplitChoice approx_choice(const DevUpdate& u, const Box& node_
   return SplitChoice{best_axis, best_pos};
 }
 
-// Whether the approximate SAH build splits objects [b, e) under a node with box node_box
-// (partitionApproxSAH's decision, the ids not moved).
-__device__ inline bool approx_splits(const DevUpdate& u, const Box& node_box, int b, int e) {
-  const int n = e - b;
-  if (n <= 1) return false;
-  if (n == 2) return pair_splits(u, node_box, b);
-  return approx_choice(u, node_box, b, e).axis != -1;
-}
-
 // partitionApproxSAH (DynBVH.cc:2125-2275)
 __device__ inline Partition partition_approx(const DevUpdate& u, int node, int b, int e) {
   Partition p;
@@ -379,22 +370,11 @@ __device__ inline void build_loop(const DevUpdate& u, Pairs& pairs, int start, i
   }
 }
 
-// A leaf bit of the rotating update, outside the reference's two (bit 0 isLeftCheaper,
-// bit 1 isLargeSubtree): set by the leaf pass when the approximate SAH keeps the leaf
-// whole, so that its parent's rotation need not read the objects again, and cleared there.
-// Every marked leaf is still its parent's child when the parent rotates (rotations below
-// do not reach a sibling leaf), so no mark outlives the frame.
-constexpr uint8_t kKeepLeaf = 0x80;
-
 // build(leaf, begin, end, useApproximateBuild = true, nodeAlreadyExists = true), as
 // rotateNode calls it on a leaf child (DynBVH.cc:3057-3069): the leaf stays unless the
 // approximate SAH splits it.
 __device__ inline void split_leaf(const DevUpdate& u, int leaf) {
   RefNode* refs = u.refs;
-  if (refs[leaf].bits & kKeepLeaf) {  // decided by the leaf pass on the same objects and box
-    refs[leaf].bits &= (uint8_t)~kKeepLeaf;
-    return;
-  }
   const int b0 = refs[leaf].child, e0 = b0 + refs[leaf].children;
   const Partition p0 = partition_approx(u, leaf, b0, e0);
   if (p0.axis == -1) return;

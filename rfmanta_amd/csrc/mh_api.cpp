@@ -910,7 +910,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   const size_t nrec = s->refs.bytes / sizeof(mh::RefNode);
   // Work arrays, allocated at the first update.
   if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->hist.reserve(mh::kFrontierWords * 4)) ||
-      (rc = s->counters.reserve(16)))
+      (rc = s->counters.reserve(16)) ||
+      (rc = s->split_list.reserve((size_t)std::max<uint32_t>(1u, T) * 4)))
     return rc;
   if (!s->host_ctr) {
     void* h = nullptr;
@@ -920,8 +921,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   }
   unsigned* nfree = s->counters.as<unsigned>();
   int* err = reinterpret_cast<int*>(nfree + 1);
-  const unsigned init[2] = {s->next_free, 0u};
-  HIP_OK(hipMemcpyAsync(nfree, init, 8, hipMemcpyHostToDevice, st));
+  const unsigned init[3] = {s->next_free, 0u, 0u};  // next_free, err, split list entries
+  HIP_OK(hipMemcpyAsync(nfree, init, sizeof(init), hipMemcpyHostToDevice, st));
   mh::DevUpdate u{};
   u.refs = s->refs.as<mh::RefNode>();
   u.pairs = s->pairs.as<mh::PairNode>();
@@ -936,6 +937,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   u.idx = s->d_idx.as<uint32_t>();
   u.next_free = nfree;
   u.err = err;
+  u.split_list = s->split_list.as<int32_t>();
+  u.split_count = nfree + 2;
   u.n_records = (int32_t)nrec;
   u.n_objects = (int32_t)T;
   // largeSubtreeSize = nodes.size() / (20 * numProcs) (DynBVH.cc:1776)

@@ -177,6 +177,8 @@ struct DevUpdate {
   const uint32_t* idx;
   unsigned* next_free;   // records in use (nextFree)
   int* err;              // set when the record array is full
+  int32_t* split_list;   // update: leaves whose parent's rotation may split them
+  unsigned* split_count; // update: entries of split_list (cleared by the host)
   int32_t n_records;     // capacity, 2T
   int32_t n_objects;     // T
   uint32_t large;        // largeSubtreeSize

@@ -11,8 +11,9 @@
 // thread per node:
 //   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030), for the device
 //                    build; the update forms them from the vertices where it needs them
-//   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097), and whether each leaf stays whole
-//                    when its parent's rotation tries to split it
+//   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097), and a list of the leaves the
+//                    parent's rotation may split
+//   k_split_leaves   those splits, ahead of the rotations (same subtrees)
 //   k_upd_level      one launch per depth: union of the child boxes, then rotateNode
 //   k_upd_top        the same for the small depths at the top, in one workgroup
 //   k_frontier       one launch per depth, top down: the next frame's level lists and
@@ -76,8 +77,8 @@ __device__ void rotate_node(const DevUpdate& u, int node) {
   float* costs = u.costs;
   uint32_t* ss = u.ssize;
   const int L = refs[node].child, R = L + 1;
-  if (refs[L].children > 1) split_leaf(u, L);
-  if (refs[R].children > 1) split_leaf(u, R);
+  // rotateNode first splits leaf children of more than one object with the approximate SAH
+  // (DynBVH.cc:3057-3069); k_split_leaves has done that for every leaf before the levels.
   const int LL = refs[L].child, LR = LL + 1, RL = refs[R].child, RR = RL + 1;
   const bool l_inner = refs[L].children == 0, r_inner = refs[R].children == 0;
   const float nodeArea = area(rbox(refs[node]));
@@ -257,21 +258,45 @@ __global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, const int32_
   const int first = nd.child, cnt = nd.children;
   Box b;
   reset(b);
-  // The parent's rotation will ask whether this leaf splits: decide it now, from the same
-  // object boxes (a root leaf has no parent to ask, a one-object leaf never splits).
-  bool keep;
-  if (cnt == 2) {  // the common leaf: both boxes once, for the leaf box and partition2Objs
+  // The parent's rotation tries to split this leaf with the approximate SAH. Two objects:
+  // decide it now, from the same object boxes (partition2Objs); more: leave the binned
+  // decision to the split pass, whose registers it needs. A root leaf has no parent, a
+  // one-object leaf never splits.
+  bool maybe;
+  if (cnt == 2) {  // the common leaf: both boxes once, for the leaf box and the decision
     const Box o0 = obj_box(u, u.ids[first]), o1 = obj_box(u, u.ids[first + 1]);
     extend(b, o0);
     extend(b, o1);
-    keep = !pair_splits(b, o0, o1);
+    maybe = pair_splits(b, o0, o1);
   } else {
     for (int k = 0; k < cnt; ++k) extend(b, obj_box(u, u.ids[first + k]));
-    keep = cnt > 2 && !approx_splits(u, b, first, first + cnt);
+    maybe = cnt > 2;
   }
   set_box(nd, b);
   nd.axis = 0;
-  if (rec != 0 && keep) nd.bits |= kKeepLeaf;
+  // the candidates go to the split list, one atomic per wave
+  const bool cand = rec != 0 && maybe;
+  const unsigned long long m = __ballot(cand);
+  if (m) {
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(u.split_count, (unsigned)__popcll(m));
+    base = (unsigned)__shfl((int)base, leader);
+    if (cand) u.split_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = rec;
+  }
+}
+
+// The leaf splits of this frame's rotations, ahead of them, for the leaves the leaf pass
+// listed (split_leaf decides again and keeps a leaf the SAH does not split). rotateNode
+// splits a leaf child when its parent rotates (DynBVH.cc:3057-3069), from the leaf's own
+// objects and box only; nothing before that rotation changes the leaf (rotations below the
+// parent do not reach a sibling leaf), and the nodes a split creates are in none of this
+// frame's level lists, so splitting the listed leaves here builds the same subtrees. The
+// split code's registers stay out of the level kernels.
+__global__ __launch_bounds__(kBlock) void k_split_leaves(DevUpdate u) {
+  const unsigned n = *u.split_count;
+  for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    split_leaf(u, u.split_list[i]);
 }
 
 // One inner node of the update: the union of its child boxes, then rotateNode.
@@ -679,9 +704,12 @@ int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leav
                        const int32_t* levels, const int64_t* level_off, int n_levels,
                        void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (n_leaves > 0)
+  if (n_leaves > 0) {
     hipLaunchKernelGGL(k_upd_leaves, dim3(grid(n_leaves)), dim3(kBlock), 0, s, u, leaves,
                        n_leaves);
+    const unsigned g = grid(n_leaves);  // the list holds at most n_leaves entries
+    hipLaunchKernelGGL(k_split_leaves, dim3(g < 1024 ? g : 1024), dim3(kBlock), 0, s, u);
+  }
   TopLevels t;
   const int top = top_levels(level_off, n_levels, t);
   for (int d = n_levels - 1; d >= top; --d) {
