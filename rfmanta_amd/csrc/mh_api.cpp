@@ -911,7 +911,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   // Work arrays, allocated at the first update.
   if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->hist.reserve(mh::kFrontierWords * 4)) ||
       (rc = s->counters.reserve(16)) ||
-      (rc = s->split_list.reserve((size_t)std::max<uint32_t>(1u, T) * 4)))
+      (rc = s->split_list.reserve((size_t)std::max<uint32_t>(1u, T) * 4)) ||
+      (rc = s->levels_aux.reserve((size_t)std::max<uint32_t>(1u, T) * 8)))
     return rc;
   if (!s->host_ctr) {
     void* h = nullptr;
@@ -947,7 +948,7 @@ int mh_scene_update(mh_scene* s, void* stream) {
   HIP_OK((hipError_t)mh::launch_update_tree(u, s->leaves.as<int32_t>(), s->n_leaves,
                                             s->levels.as<int32_t>(), s->level_off.data(),
                                             (int)s->level_off.size() - 1, st));
-  mh::DevRefit r{};
+  mh::DevRefit r{};  // triangle records in slot order from the current vertices
   r.tris = s->tris.as<float>();
   r.obj_ids = s->ids.as<int32_t>();
   r.verts = s->d_verts.as<float>();
@@ -978,6 +979,7 @@ int mh_scene_update(mh_scene* s, void* stream) {
     const int n_top = std::max(1, std::min(mh::top_levels(s->level_off.data(), prev_levels, t),
                                            d_end));
     HIP_OK((hipError_t)mh::launch_frontier_top(u, fr, s->levels.as<int32_t>(),
+                                               s->levels_aux.as<int2>(),
                                                s->leaves.as<int32_t>(), cap, n_top, st));
     d = n_top;
   }
@@ -988,6 +990,7 @@ int mh_scene_update(mh_scene* s, void* stream) {
       const int64_t prev = d < prev_levels ? s->level_off[d + 1] - s->level_off[d] : 256;
       const unsigned blocks = (unsigned)std::min<int64_t>(2048, (prev + prev / 4 + 255) / 256);
       HIP_OK((hipError_t)mh::launch_frontier(u, fr, s->levels.as<int32_t>(),
+                                             s->levels_aux.as<int2>(),
                                              s->leaves.as<int32_t>(), cap, d, blocks, st));
     }
     if (inner_root)

@@ -204,12 +204,13 @@ int launch_depth_scatter(const DevUpdate& u, const int32_t* dep, unsigned* curso
 // [kMaxDepth + 1], list offsets from [kMaxDepth + 2]). launch_frontier_top clears them and
 // runs depths [0, n_top) (1 <= n_top <= kTopLevels) in one workgroup; launch_frontier(d)
 // then lists depth d + 1 (d < kMaxDepth - 1) and the leaf children of depth d, in
-// `blocks` workgroups. Each list holds at most cap entries.
+// `blocks` workgroups. Each list holds at most cap entries; aux (cap entries) holds each
+// level entry's first child and record word for the next depth.
 constexpr int kFrontierWords = 2 * kMaxDepth + 3;
-int launch_frontier_top(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
-                        unsigned cap, int n_top, void* stream);
-int launch_frontier(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
-                    unsigned cap, int d, unsigned blocks, void* stream);
+int launch_frontier_top(const DevUpdate& u, unsigned* fr, int32_t* levels, int2* aux,
+                        int32_t* leaves, unsigned cap, int n_top, void* stream);
+int launch_frontier(const DevUpdate& u, unsigned* fr, int32_t* levels, int2* aux,
+                    int32_t* leaves, unsigned cap, int d, unsigned blocks, void* stream);
 
 // refit.hip
 int launch_prim_setup(const DevRefit& r, void* stream);  // tris from verts (all slots)

@@ -96,5 +96,20 @@ __device__ __forceinline__ void wald_record(const Tri& t, float* o) {
   o[11] = 0.f;
 }
 
+// The 48-byte record of triangle t at slot `slot` of the triangle array (prim_kind 1 =
+// Wald, else Kensler-Shirley), as three 16-byte stores.
+__device__ __forceinline__ void store_tri_record(float* tris, int prim_kind, int64_t slot,
+                                                 const Tri& t) {
+  float o[12];
+  if (prim_kind == 1)
+    wald_record(t, o);
+  else
+    ks_record(t, o);
+  float4* dst = reinterpret_cast<float4*>(tris + 12 * slot);
+  dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+  dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+  dst[2] = make_float4(o[8], o[9], o[10], o[11]);
+}
+
 }  // namespace
 }  // namespace mh

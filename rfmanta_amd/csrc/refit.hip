@@ -32,15 +32,7 @@ __device__ __forceinline__ float pick_hi(float a, float b) { return a > b ? a : 
 
 
 __device__ __forceinline__ void write_record(const DevRefit& r, int slot, const Tri& t) {
-  float o[12];
-  if (r.prim_kind == 1)
-    wald_record(t, o);
-  else
-    ks_record(t, o);
-  float4* dst = reinterpret_cast<float4*>(r.tris + 12 * (size_t)slot);
-  dst[0] = make_float4(o[0], o[1], o[2], o[3]);
-  dst[1] = make_float4(o[4], o[5], o[6], o[7]);
-  dst[2] = make_float4(o[8], o[9], o[10], o[11]);
+  store_tri_record(r.tris, r.prim_kind, slot, t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_prim_setup(DevRefit r) {

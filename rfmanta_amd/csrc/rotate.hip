@@ -358,15 +358,17 @@ __device__ __forceinline__ int32_t encode_rec(const DevUpdate& u, const RecWords
   return ~((child << kLeafCountBits) | c);
 }
 
-// The PairNode of inner record r (loaded as nd) from its two child records c0, c1, as four
-// 16-byte stores.
-__device__ __forceinline__ void store_pair(const DevUpdate& u, int64_t r, const RecWords& nd,
+// The word of a record holding axis (byte 0), bits (byte 1) and children (bytes 2-3).
+__device__ __forceinline__ int rec_word(const RecWords& w) { return __float_as_int(w.b.w); }
+
+// The PairNode of inner record r (first child c, word `word`) from its two child records
+// c0, c1, as four 16-byte stores.
+__device__ __forceinline__ void store_pair(const DevUpdate& u, int64_t r, int c, int word,
                                            const RecWords& c0, const RecWords& c1) {
-  float4* P = reinterpret_cast<float4*>(u.pairs + ((rec_child(nd) - 1) >> 1));
+  float4* P = reinterpret_cast<float4*>(u.pairs + ((c - 1) >> 1));
   P[0] = make_float4(c0.a.x, c0.a.w, c0.a.y, c0.b.x);  // child 0: lo.x hi.x lo.y hi.y
   P[1] = make_float4(c0.a.z, c0.b.y, c1.a.x, c1.a.w);  // lo.z hi.z | child 1: lo.x hi.x
   P[2] = make_float4(c1.a.y, c1.b.x, c1.a.z, c1.b.y);  // lo.y hi.y lo.z hi.z
-  const int word = __float_as_int(nd.b.w);
   reinterpret_cast<int4*>(P)[3] =
       make_int4(encode_rec(u, c0), encode_rec(u, c1), (word & 3) | (((word >> 8) & 1) << 2), (int)r);
 }
@@ -378,13 +380,15 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
   const RecWords nd = load_rec(u.refs, r);
   if (rec_children(nd) != 0) return;
   const int c = rec_child(nd);
-  store_pair(u, r, nd, load_rec(u.refs, c), load_rec(u.refs, c + 1));
+  store_pair(u, r, c, rec_word(nd), load_rec(u.refs, c), load_rec(u.refs, c + 1));
 }
 
 // Level lists of the updated tree, top down, and its PairNodes. fr holds kFrontierWords
 // counters: fr[d] inner records of depth d (d < kMaxDepth - 1), fr[kMaxDepth] leaves,
 // fr[kMaxDepth + 1] set when a list would pass its capacity, and from fr + kMaxDepth + 2
-// the list offsets: depth d is levels[off[d], off[d] + fr[d]). The root is an inner record.
+// the list offsets: depth d is levels[off[d], off[d] + fr[d]). Beside each list entry,
+// aux holds its record's first child and word, copied from the record when its parent
+// listed it, so that no depth loads its own records again. The root is an inner record.
 //
 // One workgroup's share of depth d, whose records are levels[first, first + n): items
 // i_begin, i_begin + stride, ... Each record writes the PairNode of its two children and
@@ -394,9 +398,9 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_pairs(DevUpdate u) {
 // first lane. Order inside a list is free: the nodes of one depth are independent.
 template <class Reserve>
 __device__ __forceinline__ void frontier_depth(const DevUpdate& u, unsigned* fr, int32_t* levels,
-                                               int32_t* leaves, unsigned cap, unsigned first,
-                                               unsigned n, unsigned i_begin, unsigned stride,
-                                               Reserve reserve) {
+                                               int2* aux, int32_t* leaves, unsigned cap,
+                                               unsigned first, unsigned n, unsigned i_begin,
+                                               unsigned stride, Reserve reserve) {
   __shared__ unsigned w_in[kBlock / 64], w_lf[kBlock / 64];
   __shared__ unsigned b_in, b_lf;
   const unsigned next = first + n;
@@ -413,14 +417,17 @@ __device__ __forceinline__ void frontier_depth(const DevUpdate& u, unsigned* fr,
     const bool live = i < n && first + i < cap;  // past cap only after an overflow (flagged)
     int c = 0;
     bool in0 = false, in1 = false;
+    int2 ax0 = make_int2(0, 0), ax1 = make_int2(0, 0);
     if (live) {
       const int r = levels[first + i];
-      const RecWords nd = load_rec(u.refs, r);
-      c = rec_child(nd);
+      const int2 ax = aux[first + i];
+      c = ax.x;
       const RecWords c0 = load_rec(u.refs, c), c1 = load_rec(u.refs, c + 1);
-      store_pair(u, r, nd, c0, c1);
+      store_pair(u, r, c, ax.y, c0, c1);
       in0 = rec_children(c0) == 0;
       in1 = rec_children(c1) == 0;
+      ax0 = make_int2(rec_child(c0), rec_word(c0));
+      ax1 = make_int2(rec_child(c1), rec_word(c1));
     }
     const int k_in = live ? (int)in0 + (int)in1 : 0, k_lf = live ? 2 - k_in : 0;
     const unsigned long long i1 = __ballot(k_in >= 1), i2 = __ballot(k_in == 2);
@@ -448,14 +455,18 @@ __device__ __forceinline__ void frontier_depth(const DevUpdate& u, unsigned* fr,
     if (live) {
       unsigned pi = next + b_in + w_in[wave] + (unsigned)(__popcll(i1 & below) + __popcll(i2 & below));
       unsigned pl = b_lf + w_lf[wave] + (unsigned)(__popcll(l1 & below) + __popcll(l2 & below));
-      if (in0)
+      if (in0) {
+        if (pi < cap) aux[pi] = ax0;
         put(levels, pi++, c);
-      else
+      } else {
         put(leaves, pl++, c);
-      if (in1)
+      }
+      if (in1) {
+        if (pi < cap) aux[pi] = ax1;
         put(levels, pi, c + 1);
-      else
+      } else {
         put(leaves, pl, c + 1);
+      }
     }
     __syncthreads();  // w_in, w_lf, b_in, b_lf are rewritten by the next iteration
   }
@@ -466,7 +477,8 @@ __device__ __forceinline__ void frontier_depth(const DevUpdate& u, unsigned* fr,
 // the list entries stored before it are seen after it) and leaves fr[0..n_top], the
 // offsets up to depth n_top and the leaf count for the launches of the deeper depths.
 __global__ __launch_bounds__(kBlock) void k_frontier_top(DevUpdate u, unsigned* fr, int32_t* levels,
-                                                         int32_t* leaves, unsigned cap, int n_top) {
+                                                         int2* aux, int32_t* leaves, unsigned cap,
+                                                         int n_top) {
   __shared__ unsigned s_cnt[kTopLevels + 1], s_off[kTopLevels + 1], s_leaf;
   for (int i = threadIdx.x; i < kFrontierWords; i += kBlock) fr[i] = 0u;
   if (threadIdx.x == 0) {
@@ -474,6 +486,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier_top(DevUpdate u, unsigned* 
     s_off[0] = 0u;
     s_leaf = 0u;
     levels[0] = 0;
+    const RecWords root = load_rec(u.refs, 0);
+    aux[0] = make_int2(rec_child(root), rec_word(root));
   }
   __syncthreads();
   for (int d = 0; d < n_top; ++d) {
@@ -484,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_top(DevUpdate u, unsigned* 
       s_cnt[d + 1] = 0u;
     }
     __syncthreads();
-    frontier_depth(u, fr, levels, leaves, cap, first, n, 0u, kBlock,
+    frontier_depth(u, fr, levels, aux, leaves, cap, first, n, 0u, kBlock,
                    [&](unsigned t_in, unsigned t_lf, unsigned& bi, unsigned& bl) {
                      bi = s_cnt[d + 1];
                      s_cnt[d + 1] = bi + t_in;
@@ -505,11 +519,13 @@ __global__ __launch_bounds__(kBlock) void k_frontier_top(DevUpdate u, unsigned* 
 
 // Depth d (>= the depths of k_frontier_top) over the whole grid, counters in fr.
 __global__ __launch_bounds__(kBlock) void k_frontier(DevUpdate u, unsigned* fr, int32_t* levels,
-                                                     int32_t* leaves, unsigned cap, int d) {
+                                                     int2* aux, int32_t* leaves, unsigned cap,
+                                                     int d) {
   unsigned* off = fr + kMaxDepth + 2;
   const unsigned n = fr[d], first = off[d];
   if (blockIdx.x == 0 && threadIdx.x == 0) off[d + 1] = first + n;
-  frontier_depth(u, fr, levels, leaves, cap, first, n, blockIdx.x * kBlock, gridDim.x * kBlock,
+  frontier_depth(u, fr, levels, aux, leaves, cap, first, n, blockIdx.x * kBlock,
+                 gridDim.x * kBlock,
                  [&](unsigned t_in, unsigned t_lf, unsigned& bi, unsigned& bl) {
                    bi = t_in ? atomicAdd(fr + d + 1, t_in) : 0u;
                    bl = t_lf ? atomicAdd(fr + kMaxDepth, t_lf) : 0u;
@@ -721,19 +737,19 @@ int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leav
   return (int)hipGetLastError();
 }
 
-int launch_frontier_top(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
-                        unsigned cap, int n_top, void* stream) {
+int launch_frontier_top(const DevUpdate& u, unsigned* fr, int32_t* levels, int2* aux,
+                        int32_t* leaves, unsigned cap, int n_top, void* stream) {
   if (n_top < 1 || n_top > kTopLevels || n_top >= kMaxDepth - 1) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_frontier_top, dim3(1), dim3(kBlock), 0,
-                     reinterpret_cast<hipStream_t>(stream), u, fr, levels, leaves, cap, n_top);
+                     reinterpret_cast<hipStream_t>(stream), u, fr, levels, aux, leaves, cap, n_top);
   return (int)hipGetLastError();
 }
 
-int launch_frontier(const DevUpdate& u, unsigned* fr, int32_t* levels, int32_t* leaves,
-                    unsigned cap, int d, unsigned blocks, void* stream) {
+int launch_frontier(const DevUpdate& u, unsigned* fr, int32_t* levels, int2* aux,
+                    int32_t* leaves, unsigned cap, int d, unsigned blocks, void* stream) {
   if (d < 0 || d >= kMaxDepth - 1) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_frontier, dim3(blocks ? blocks : 1), dim3(kBlock), 0,
-                     reinterpret_cast<hipStream_t>(stream), u, fr, levels, leaves, cap, d);
+                     reinterpret_cast<hipStream_t>(stream), u, fr, levels, aux, leaves, cap, d);
   return (int)hipGetLastError();
 }
 
