@@ -13,9 +13,11 @@
 //      device by a kernel queued behind them, and their chunks join later tables) and
 //      queues the table upload and one persistent trace; scenes have their own streams, so
 //      their launches overlap, and the next table is built while a launch runs;
-//   3. for a request whose rays are all queued, reserves a range of the pinned host arena
-//      and queues the result copies on a copy stream behind an event of its last launch,
-//      so copies overlap the next launches;
+//   3. for a request whose primary rays are all queued and whose secondary rays follow,
+//      reserves a range of the pinned host arena and queues the primary results' copies on
+//      a copy stream behind an event of its last primary launch, so they overlap the
+//      secondary launches; for a request whose rays are all queued, queues the remaining
+//      copies behind an event of its last launch, so they overlap the next launches;
 //   4. retires launches and copies whose events have completed; a request is finished, and
 //      its device range freed, when its copies are done.
 // mh_server_wait copies a finished request's results out of the host arena and frees that
@@ -130,6 +132,8 @@ struct Request {
   bool cancelled = false;        // mh_server_cancel: no further chunks are handed out
   hipEvent_t retire = nullptr;   // completes when the request's copies (or work) are done
   hipEvent_t after = nullptr;    // its last launch, which the copies wait for
+  hipEvent_t prim_after = nullptr;  // its last primary launch (requests with secondary rays)
+  bool prim_copied = false;         // primary results already queued for copying
   // results in the host arena
   int32_t* h_prim = nullptr;
   float* h_t = nullptr;
@@ -379,6 +383,10 @@ int mh_server::queue_scene(int k, const std::vector<std::shared_ptr<Request>>& w
     if (r->prim.next < r->n) continue;
     r->primary_done = true;
     if (!r->sec.empty()) {
+      // the primary results are final once the launches queued so far have run: their
+      // copies can overlap the secondary rays' launches
+      if (int rc = event(r->prim_after)) return rc;
+      HIP_OK(hipEventRecord(r->prim_after, P.stream));
       const mh_scene* sc = scenes[k];
       mh::SecondaryArgs a{};
       const mh_rays& p = r->prim.rays;
@@ -442,7 +450,9 @@ int mh_server::step() {
   moved = moved || traced > 0;
   // Requests with all their rays queued: host ranges and result copies behind their
   // scene's launches; cancelled requests: an event behind the work already queued.
-  std::vector<std::shared_ptr<Request>> to_copy;
+  // Requests whose primary launches are all queued and whose secondary rays are not: a host
+  // range and the primary copies behind those launches, ahead of the rest.
+  std::vector<std::shared_ptr<Request>> to_copy, prim_copy;
   {
     std::lock_guard<std::mutex> lk(mu);
     st.rays += traced;
@@ -452,15 +462,31 @@ int mh_server::step() {
         to_copy.push_back(r);
         continue;
       }
-      if (!r->handed_all) continue;
-      const size_t hb = host_bytes(r->n, opts.shadows != 0, opts.bounces != 0);
-      size_t off;
-      if (!host_arena->alloc(hb, off)) continue;  // waits for collections
-      r->host_off = off;
-      r->host_bytes = hb;
-      r->has_host = true;
-      to_copy.push_back(r);
+      const bool early = r->prim_after && !r->prim_copied && !r->handed_all;
+      if (!r->handed_all && !early) continue;
+      if (!r->has_host) {
+        const size_t hb = host_bytes(r->n, opts.shadows != 0, opts.bounces != 0);
+        size_t off;
+        if (!host_arena->alloc(hb, off)) continue;  // waits for collections
+        r->host_off = off;
+        r->host_bytes = hb;
+        r->has_host = true;
+      }
+      (early ? prim_copy : to_copy).push_back(r);
     }
+  }
+  for (auto& r : prim_copy) {
+    HIP_OK(hipStreamWaitEvent(copy_stream, r->prim_after, 0));
+    const int64_t n = r->n;
+    Carver c{host_mem + r->host_off};
+    r->h_prim = c.take<int32_t>(n);
+    r->h_t = c.take<float>(n);
+    HIP_OK(hipMemcpyAsync(r->h_prim, r->prim.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                          copy_stream));
+    HIP_OK(hipMemcpyAsync(r->h_t, r->prim.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                          copy_stream));
+    r->prim_copied = true;
+    moved = true;
   }
   for (auto& r : to_copy) {
     hipStream_t s = per[r->scene]->stream;
@@ -468,7 +494,18 @@ int mh_server::step() {
     if (int rc = event(after)) return rc;
     HIP_OK(hipEventRecord(after, s));
     if (r->status != MH_OK || r->cancelled) {
-      r->retire = after;  // its buffers are free once the queued work has passed
+      // its buffers are free once the queued work has passed, and its host range once the
+      // primary copies already queued for it are done
+      if (r->prim_copied) {
+        HIP_OK(hipStreamWaitEvent(copy_stream, after, 0));
+        hipEvent_t copied;
+        if (int rc = event(copied)) return rc;
+        HIP_OK(hipEventRecord(copied, copy_stream));
+        r->after = after;
+        r->retire = copied;
+      } else {
+        r->retire = after;
+      }
       moved = true;
       continue;
     }
@@ -478,10 +515,12 @@ int mh_server::step() {
     Carver c{host_mem + r->host_off};
     r->h_prim = c.take<int32_t>(n);
     r->h_t = c.take<float>(n);
-    HIP_OK(hipMemcpyAsync(r->h_prim, r->prim.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost,
-                          copy_stream));
-    HIP_OK(hipMemcpyAsync(r->h_t, r->prim.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost,
-                          copy_stream));
+    if (!r->prim_copied) {
+      HIP_OK(hipMemcpyAsync(r->h_prim, r->prim.hits.prim, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                            copy_stream));
+      HIP_OK(hipMemcpyAsync(r->h_t, r->prim.hits.t, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                            copy_stream));
+    }
     for (auto& x : r->sec) {
       if (x.any) {
         r->h_occ = c.take<uint8_t>(n);
@@ -513,6 +552,8 @@ int mh_server::step() {
     r->retire = nullptr;
     if (r->after) free_events.push_back(r->after);
     r->after = nullptr;
+    if (r->prim_after) free_events.push_back(r->prim_after);
+    r->prim_after = nullptr;
     retired.push_back(r);
   }
   if (!retired.empty()) {
