@@ -309,6 +309,9 @@ typedef struct mh_launch_config {
                         rays with no counter (whole-wave mode only) */
   int xcd_remap;     /* 1 = give each XCD (every 8th workgroup) a contiguous share of the
                         rays so that neighbouring rays share an L2 */
+  int reverse;       /* 1 = hand rays out from the end of the table (last batch first, in
+                        chunks whose rays ascend), so that a table whose costly batches come
+                        last still ends on short rays; results are the same */
 } mh_launch_config;
 
 typedef struct mh_frame mh_frame;

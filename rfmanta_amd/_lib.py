@@ -59,7 +59,7 @@ class Batch(C.Structure):
 class LaunchConfig(C.Structure):
     _fields_ = [("layout", C.c_int), ("stack_lds", C.c_int), ("block", C.c_int),
                 ("blocks_per_cu", C.c_int), ("refill", C.c_int), ("chunk", C.c_int),
-                ("xcd_remap", C.c_int)]
+                ("xcd_remap", C.c_int), ("reverse", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -67,7 +67,7 @@ class LaunchConfig(C.Structure):
     @classmethod
     def from_dict(cls, d: dict) -> "LaunchConfig":
         return cls(d["layout"], d["stack_lds"], d["block"], d["blocks_per_cu"], d["refill"],
-                   d.get("chunk", 64), d.get("xcd_remap", 0))
+                   d.get("chunk", 64), d.get("xcd_remap", 0), d.get("reverse", 0))
 
 
 MH_LAYOUT_PAIR = 0

@@ -15,8 +15,10 @@
 //     ray has finished takes the next number, between traversal rounds, as soon as at
 //     least `refill` lanes of its wave are idle, so lanes do not wait for the wave's
 //     slowest ray (ballot + popcount hand-out, no LDS traffic).
-// The last wave to leave resets the counters, so consecutive launches (and graph
-// replays) need no clearing pass.
+// With `reverse` the chunks are handed out from the end of the table (last batch first,
+// rays ascending inside each chunk), so that a table listing its costly batches last still
+// ends on short rays. The last wave to leave resets the counters, so consecutive launches
+// (and graph replays) need no clearing pass.
 //
 // Three node layouts, all giving the reference's one-ray results and visit order
 // (DynBVH::intersectNode on a one-ray packet, Model/Groups/DynBVH.cc:274-404; the
@@ -264,7 +266,7 @@ template <int PRIM, int LAYOUT, int S, int BLOCK, int MODE, int KIND>
 __global__ __launch_bounds__(BLOCK) __attribute__((
     amdgpu_waves_per_eu(MODE == kModeLane && KIND == kKindMixed ? 6 : 7, 8))) void k_trace_batch(
     DevScene sc, const BatchTable* __restrict__ table, unsigned long long* ctr, int2* spill,
-    int64_t spill_stride, int nb_launch, int refill, int chunk, int xcd_remap) {
+    int64_t spill_stride, int nb_launch, int refill, int chunk, int xcd_remap, int reverse) {
   __shared__ unsigned long long s_ring[S * BLOCK];
   extern __shared__ int64_t s_start[];  // nb entries (dynamic LDS)
   // The dynamic LDS holds nb_launch offsets; a table with more entries is cut to that.
@@ -297,8 +299,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
     const int64_t static_part = n_waves * C;
     int64_t next = wave_id * C;
     while (next < total) {
-      const int64_t stop = next + C < total ? next + C : total;
-      int64_t pos = next;
+      // chunk [next, next + C) of the hand-out order; reversed, the chunk of ray numbers
+      // that many from the end (rays still ascend inside it)
+      int64_t pos, stop;
+      if (reverse) {
+        stop = total - next;
+        pos = stop - C > 0 ? stop - C : 0;
+      } else {
+        pos = next;
+        stop = next + C < total ? next + C : total;
+      }
       while (pos < stop) {
         const int bi = __builtin_amdgcn_readfirstlane(find_batch(s_start, nb, pos));
         const BatchDev& B = batches[bi];  // uniform: scalar loads
@@ -349,6 +359,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((
         if (n_idle > avail) {
           int64_t base, end;
           grab_chunk(ctr, C, total, xcd_remap != 0, xr, lane, base, end);
+          if (reverse && base >= 0) {  // the same chunk counted from the end of the table
+            const int64_t b = total - end;
+            end = total - base;
+            base = b;
+          }
           if (base < 0) {
             exhausted = true;
             pool_next = pool_end = total;
@@ -415,17 +430,17 @@ int launch_w(const DevScene& sc, const BatchLaunch& bl, hipStream_t s) {
     case 8:
       hipLaunchKernelGGL((k_trace_batch<PRIM, LAYOUT, 8, BLOCK, MODE, KIND>), g, b, lds, s, sc,
                          bl.table, bl.counters, sp, bl.spill_stride, bl.nb, bl.refill, bl.chunk,
-                         bl.xcd_remap);
+                         bl.xcd_remap, bl.reverse);
       break;
     case 32:
       hipLaunchKernelGGL((k_trace_batch<PRIM, LAYOUT, 32, BLOCK, MODE, KIND>), g, b, lds, s, sc,
                          bl.table, bl.counters, sp, bl.spill_stride, bl.nb, bl.refill, bl.chunk,
-                         bl.xcd_remap);
+                         bl.xcd_remap, bl.reverse);
       break;
     default:
       hipLaunchKernelGGL((k_trace_batch<PRIM, LAYOUT, 16, BLOCK, MODE, KIND>), g, b, lds, s, sc,
                          bl.table, bl.counters, sp, bl.spill_stride, bl.nb, bl.refill, bl.chunk,
-                         bl.xcd_remap);
+                         bl.xcd_remap, bl.reverse);
       break;
   }
   return (int)hipGetLastError();

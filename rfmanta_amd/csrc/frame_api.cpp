@@ -63,6 +63,7 @@ int check_launch_config(const mh_launch_config& c) {
   if (c.chunk == 0 && c.refill < 64)
     return fail(MH_E_INVALID, "chunk 0 (static stride) needs refill 64");
   if (c.xcd_remap != 0 && c.xcd_remap != 1) return fail(MH_E_INVALID, "xcd_remap %d", c.xcd_remap);
+  if (c.reverse != 0 && c.reverse != 1) return fail(MH_E_INVALID, "reverse %d", c.reverse);
   const size_t lds = (size_t)c.stack_lds * c.block * 8 + 256 * 8;
   if (lds > 65536) return fail(MH_E_UNSUPPORTED, "LDS per workgroup %zu > 64 KiB", lds);
   return MH_OK;
@@ -72,7 +73,7 @@ int check_launch_config(const mh_launch_config& c) {
 
 namespace {
 
-const mh_launch_config kDefault = {MH_LAYOUT_PAIR, 8, 256, 16, 64, 256, 1};
+const mh_launch_config kDefault = {MH_LAYOUT_PAIR, 8, 256, 16, 64, 256, 1, 0};
 
 
 // Kernel specialisation for a table: closest-hit only, any-hit only, or mixed.
@@ -127,7 +128,7 @@ int prepare(mh_frame* f, const mh_launch_config& c) {
   if (f->spill.p != old_spill) f->drop_graph();  // a captured launch holds the old area
   if (c.layout != f->cfg.layout || c.stack_lds != f->cfg.stack_lds || c.block != f->cfg.block ||
       c.blocks_per_cu != f->cfg.blocks_per_cu || c.refill != f->cfg.refill ||
-      c.chunk != f->cfg.chunk || c.xcd_remap != f->cfg.xcd_remap)
+      c.chunk != f->cfg.chunk || c.xcd_remap != f->cfg.xcd_remap || c.reverse != f->cfg.reverse)
     f->drop_graph();
   f->cfg = c;
   f->blocks = blocks;
@@ -169,6 +170,7 @@ int launch(mh_frame* f, void* stream) {
   bl.refill = f->cfg.refill;
   bl.chunk = f->cfg.chunk;
   bl.xcd_remap = f->cfg.xcd_remap;
+  bl.reverse = f->cfg.reverse;
   bl.kinds = kinds_of(f->nb, f->n_any);
   HIP_OK((hipError_t)mh::launch_trace_batch(f->scene->dev(), bl, stream));
   return MH_OK;
@@ -389,18 +391,18 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
   if (cache_path) cache = read_cache(cache_path);
   if (cache_path && !force && cache.count(key)) {
     const std::string& obj = cache[key];
-    double v[8];
-    const char* names[8] = {"layout", "stack_lds", "block", "blocks_per_cu",
-                            "refill", "chunk", "xcd_remap", "ms"};
+    double v[9];
+    const char* names[9] = {"layout", "stack_lds", "block", "blocks_per_cu",
+                            "refill", "chunk", "xcd_remap", "reverse", "ms"};
     bool ok = true;
-    for (int i = 0; i < 8; ++i) ok = ok && field(obj, names[i], v[i]);
-    if (ok) {
+    for (int i = 0; i < 9; ++i) ok = ok && field(obj, names[i], v[i]);
+    if (ok) {  // entries written before the reverse field was added are tuned again
       mh_launch_config c{(int)v[0], (int)v[1], (int)v[2], (int)v[3],
-                         (int)v[4], (int)v[5], (int)v[6]};
+                         (int)v[4], (int)v[5], (int)v[6], (int)v[7]};
       if (mh_internal::check_launch_config(c) == MH_OK) {
         if ((rc = prepare(f, c))) return rc;
         if (best_out) *best_out = c;
-        if (best_ms_out) *best_ms_out = (float)v[7];
+        if (best_ms_out) *best_ms_out = (float)v[8];
         return MH_OK;
       }
     }
@@ -460,14 +462,22 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
         t.xcd_remap = 0;
         if ((rc = trial(t))) return rc;
       }
+  // Stage 5: the hand-out order (a table that lists its costly batches last ends on its
+  // short rays when handed out from the end).
+  {
+    mh_launch_config t = best;
+    t.reverse = 1 - t.reverse;
+    if ((rc = trial(t))) return rc;
+  }
   if ((rc = prepare(f, best))) return rc;
   if (cache_path) {
     char buf[256];
     std::snprintf(buf, sizeof(buf),
                   "{\"layout\": %d, \"stack_lds\": %d, \"block\": %d, \"blocks_per_cu\": %d, "
-                  "\"refill\": %d, \"chunk\": %d, \"xcd_remap\": %d, \"ms\": %.5f}",
+                  "\"refill\": %d, \"chunk\": %d, \"xcd_remap\": %d, \"reverse\": %d, "
+                  "\"ms\": %.5f}",
                   best.layout, best.stack_lds, best.block, best.blocks_per_cu, best.refill,
-                  best.chunk, best.xcd_remap, best_ms);
+                  best.chunk, best.xcd_remap, best.reverse, best_ms);
     cache[key] = buf;
     write_cache(cache_path, cache);
   }

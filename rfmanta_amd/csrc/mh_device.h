@@ -119,6 +119,7 @@ struct BatchLaunch {
   int refill;     // idle lanes that trigger a refill, 1..63; 64 = whole-wave chunks
   int chunk;      // rays per counter grab (multiple of 64); 0 = static stride, no counter
   int xcd_remap;  // contiguous ray shares per XCD
+  int reverse;    // hand chunks out from the end of the table
   int kinds;      // 0 mixed table, 1 closest-hit batches only, 2 any-hit batches only
 };
 

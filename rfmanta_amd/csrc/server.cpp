@@ -370,6 +370,7 @@ int mh_server::queue_scene(int k, const std::vector<std::shared_ptr<Request>>& w
     bl.refill = launch.refill;
     bl.chunk = launch.chunk;
     bl.xcd_remap = launch.xcd_remap;
+    bl.reverse = launch.reverse;
     bl.kinds = n_any == 0 ? 1 : (n_any == nb ? 2 : 0);
     HIP_OK((hipError_t)mh::launch_trace_batch(scenes[k]->dev(), bl, P.stream));
     HIP_OK(hipEventRecord(P.done[slot], P.stream));

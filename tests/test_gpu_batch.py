@@ -60,6 +60,16 @@ def _device(batches):
     {"layout": 2, "stack_lds": 16, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 256},
     {"layout": 2, "stack_lds": 8, "block": 256, "blocks_per_cu": 5, "refill": 32, "chunk": 128,
      "xcd_remap": 1},
+    # handed out from the end of the table: per-lane refill (also per XCD), counter chunks,
+    # static stride (the last chunk in hand-out order is the short one at ray 0)
+    {"layout": 0, "stack_lds": 8, "block": 128, "blocks_per_cu": 6, "refill": 32, "chunk": 256,
+     "reverse": 1},
+    {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 3, "refill": 16, "chunk": 192,
+     "xcd_remap": 1, "reverse": 1},
+    {"layout": 0, "stack_lds": 16, "block": 128, "blocks_per_cu": 2, "refill": 64, "chunk": 320,
+     "reverse": 1},
+    {"layout": 1, "stack_lds": 8, "block": 256, "blocks_per_cu": 2, "refill": 64, "chunk": 0,
+     "reverse": 1},
 ])
 def test_frame_matches_oracle(oracle, small, cfg):
     v, i, batches = small
@@ -69,7 +79,7 @@ def test_frame_matches_oracle(oracle, small, cfg):
         dev = _device(batches)
         f = api.Frame(s, dev)
         f.set_config(cfg)
-        full = {"chunk": 64, "xcd_remap": 0, **cfg}
+        full = {"chunk": 64, "xcd_remap": 0, "reverse": 0, **cfg}
         assert f.config() == full
         f.launch()
         f.launch()  # a second launch checks that the counters reset themselves
