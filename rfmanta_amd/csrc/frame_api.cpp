@@ -448,20 +448,24 @@ int mh_frame_autotune(mh_frame* f, int iters, const char* cache_path, int force,
       if ((rc = trial(t))) return rc;
     }
   // Stage 4: per-lane refill, with shapes of its own: its waves stay resident and take
-  // rays in chunks, so grids of about the resident wave count (six or seven per SIMD)
-  // and chunks that leave every wave several grabs; the best ring and layout so far.
+  // rays in chunks, so grids of about the resident wave count (five to seven per SIMD)
+  // and chunks that leave every wave several grabs; the best ring and layout so far, in
+  // both hand-out orders (the order moves the best shape).
   base = best;
-  for (int r : {16, 32, 48})
-    for (int c : {64, 128, 256})
-      for (const auto& bp : {std::pair<int, int>{256, 5}, {256, 6}, {256, 7}, {128, 12}, {128, 14}}) {
-        mh_launch_config t = base;
-        t.refill = r;
-        t.chunk = c;
-        t.block = bp.first;
-        t.blocks_per_cu = bp.second;
-        t.xcd_remap = 0;
-        if ((rc = trial(t))) return rc;
-      }
+  for (int rev = 0; rev < 2; ++rev)
+    for (int r : {16, 32, 48})
+      for (int c : {64, 128, 256})
+        for (const auto& bp : {std::pair<int, int>{256, 5}, {256, 6}, {256, 7}, {128, 10},
+                               {128, 12}, {128, 14}}) {
+          mh_launch_config t = base;
+          t.refill = r;
+          t.chunk = c;
+          t.block = bp.first;
+          t.blocks_per_cu = bp.second;
+          t.xcd_remap = 0;
+          t.reverse = rev;
+          if ((rc = trial(t))) return rc;
+        }
   // Stage 5: the hand-out order (a table that lists its costly batches last ends on its
   // short rays when handed out from the end).
   {

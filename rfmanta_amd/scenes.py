@@ -257,10 +257,11 @@ def config1_scene(nx: int = 1000, nz: int = 500, seed: int = 0):
 # Launch configuration of the C1 frame (primary + bounce batches in one k_trace_batch):
 # the round refill (a lane takes a new ray when 32 of its wave's lanes are idle, rays
 # handed out 256 at a time) from the end of the table, so the incoherent bounce rays go
-# first and the frame ends on short primary rays; 12 workgroups of 128 lanes per CU, LDS
-# ring of 8, PAIR nodes. 6-7% ahead of the same schedule in table order, which was 1.5-1.7%
-# ahead of the static stride.
-C1_LAUNCH = {"layout": 0, "stack_lds": 8, "block": 128, "blocks_per_cu": 12, "refill": 32,
+# first and the frame ends on short primary rays; 10 workgroups of 128 lanes per CU (five
+# waves per SIMD), LDS ring of 8, PAIR nodes. In table order the same schedule at 12
+# workgroups per CU was 1.5-1.7% ahead of the static stride; handed out in reverse it is
+# 6-7% faster, and 2% more at 10 workgroups per CU.
+C1_LAUNCH = {"layout": 0, "stack_lds": 8, "block": 128, "blocks_per_cu": 10, "refill": 32,
              "chunk": 256, "xcd_remap": 0, "reverse": 1}
 # The static stride (24 workgroups of 256 lanes per CU), in table order.
 C1_LAUNCH_STATIC = {"layout": 0, "stack_lds": 8, "block": 256, "blocks_per_cu": 24,
