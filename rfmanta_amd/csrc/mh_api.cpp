@@ -47,6 +47,9 @@ int fail(int code, const char* fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   g_err = buf;
+  // The failure is reported here: take it off the thread's HIP error state too, or the
+  // check after this thread's next kernel launch (hipGetLastError) would report it again.
+  (void)hipGetLastError();
   return code;
 }
 

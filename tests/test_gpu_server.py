@@ -134,6 +134,24 @@ def test_admission_respects_budget(two_scenes):
     srv.close()
 
 
+def test_device_allocation_failure_is_clean(two_scenes):
+    """A server asked for more device memory than the GPU has: its hipMalloc fails and
+    mh_server_create reports MH_E_NOMEM with nothing left behind; the next server is created,
+    launches and serves a request identical to the direct path (the failed allocation's HIP
+    error does not surface in later launch checks)."""
+    (s0, _, _), _ = two_scenes
+    with pytest.raises(L.MhError) as e:
+        api.Server([s0], hbm_budget=1 << 50)
+    assert e.value.code == L.MH_E_NOMEM
+    cam = api.camera((0, 0.9, -1.1), (0, 0, 0), (0, 1, 0), 50.0, 96, 64)
+    srv = api.Server([s0], hbm_budget=64 << 20, shadows=True, light=LIGHT)
+    o = srv.wait(srv.submit(0, cam))
+    d = direct(s0, cam, shadows=True)
+    assert np.array_equal(o["prim"], d["prim"]) and np.array_equal(o["t"], d["t"])
+    assert np.array_equal(o["occluded"], d["occluded"])
+    srv.close()
+
+
 def test_host_staging_frees_on_collection(two_scenes):
     """Results wait in pinned host staging; with room for one request's results, the others
     finish on the device and complete one by one as results are collected."""
