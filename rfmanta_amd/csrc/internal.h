@@ -111,6 +111,10 @@ struct mh_scene {
   DevBuf costs, ssize, parent, obox, ocen, anc0, dep0, anc1, dep1, hist, counters;
   DevBuf split_list;  // rotating update: leaves whose parent's rotation may split them
   DevBuf levels_aux;  // rotating update: first child and word beside each level entry
+  // rotating update: per object slot, the record of the leaf whose first slot it is, else -1
+  // (filled from the leaf list after a build or load, kept by the update's level pass)
+  DevBuf leaf_at;
+  bool leaf_at_valid = false;
   DevBuf logtab;  // RTSAH table logarithm (device build)
   DevBuf normals, normal_idx, texcoords, tex_idx;  // shading attributes (mh_shade)
   // The PairNodes as four arrays of their 16-byte fields (MH_LAYOUT_SOA launches):
@@ -140,7 +144,7 @@ struct mh_scene {
            parent.bytes + obox.bytes + ocen.bytes + anc0.bytes + dep0.bytes + anc1.bytes +
            dep1.bytes + hist.bytes + counters.bytes + logtab.bytes + normals.bytes +
            normal_idx.bytes + texcoords.bytes + tex_idx.bytes + pairs_soa.bytes +
-           split_list.bytes + levels_aux.bytes;
+           split_list.bytes + levels_aux.bytes + leaf_at.bytes;
   }
   // Traversal stack entries a single ray can hold at most (one push per level).
   uint32_t stack_bound(int /*layout*/) const {

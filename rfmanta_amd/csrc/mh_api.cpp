@@ -206,6 +206,7 @@ int upload(mh_scene* s, bool rtsah = false) {
   s->root_ref = encode(0);
   s->n_inner = n_inner;
   s->n_leaves = (uint32_t)leaves.size();
+  s->leaf_at_valid = false;
   s->depth = max_depth;
   s->next_free = max_rec + 1;
   s->n_records = nrec;
@@ -636,6 +637,7 @@ int build_device(mh_scene* s, int rtsah) {
   s->next_free = n_final;
   s->n_records = (uint32_t)nrec;
   s->n_inner = (uint32_t)level_off.back();
+  s->leaf_at_valid = false;
   s->n_leaves = (uint32_t)n_leaves;
   s->depth = deepest;
   s->level_off = std::move(level_off);
@@ -915,7 +917,8 @@ int mh_scene_update(mh_scene* s, void* stream) {
   if ((rc = s->parent.reserve(nrec * 4)) || (rc = s->hist.reserve(mh::kFrontierWords * 4)) ||
       (rc = s->counters.reserve(16)) ||
       (rc = s->split_list.reserve((size_t)std::max<uint32_t>(1u, T) * 4)) ||
-      (rc = s->levels_aux.reserve((size_t)std::max<uint32_t>(1u, T) * 8)))
+      (rc = s->levels_aux.reserve((size_t)std::max<uint32_t>(1u, T) * 8)) ||
+      (rc = s->leaf_at.reserve((size_t)std::max<uint32_t>(1u, T) * 4)))
     return rc;
   if (!s->host_ctr) {
     void* h = nullptr;
@@ -943,13 +946,23 @@ int mh_scene_update(mh_scene* s, void* stream) {
   u.err = err;
   u.split_list = s->split_list.as<int32_t>();
   u.split_count = nfree + 2;
+  u.leaf_at = s->leaf_at.as<int32_t>();
+  u.tris = s->tris.as<float>();
+  u.prim_kind = s->prim_kind;
   u.n_records = (int32_t)nrec;
   u.n_objects = (int32_t)T;
   // largeSubtreeSize = nodes.size() / (20 * numProcs) (DynBVH.cc:1776)
   u.large = (uint32_t)(nrec / (20 * (size_t)s->workers));
   ++s->pairs_gen;
-  HIP_OK((hipError_t)mh::launch_update_tree(u, s->leaves.as<int32_t>(), s->n_leaves,
-                                            s->levels.as<int32_t>(), s->level_off.data(),
+  if (!s->leaf_at_valid) {  // after a build or load: the leaf starting at each slot
+    HIP_OK(hipMemsetAsync(s->leaf_at.p, 0xFF, (size_t)std::max<uint32_t>(1u, T) * 4, st));
+    HIP_OK((hipError_t)mh::launch_leaf_starts(u, s->leaves.as<int32_t>(), s->n_leaves, st));
+    s->leaf_at_valid = true;  // the level pass below keeps it for the next frame
+  }
+  // leaf pass (triangle records and leaf boxes from the current vertices), leaf splits,
+  // rotations depth by depth
+  HIP_OK((hipError_t)mh::launch_update_tree(u, s->n_leaves, s->levels.as<int32_t>(),
+                                            s->level_off.data(),
                                             (int)s->level_off.size() - 1, st));
   mh::DevRefit r{};  // triangle records in slot order from the current vertices
   r.tris = s->tris.as<float>();

@@ -180,15 +180,21 @@ struct DevUpdate {
   int* err;              // set when the record array is full
   int32_t* split_list;   // update: leaves whose parent's rotation may split them
   unsigned* split_count; // update: entries of split_list (cleared by the host)
+  int32_t* leaf_at;      // update: per slot, the leaf whose first slot it is, else -1
+  float* tris;           // update: triangle records, rewritten from the current vertices
+  int32_t prim_kind;     // update: MH_PRIM_* of the records
   int32_t n_records;     // capacity, 2T
   int32_t n_objects;     // T
   uint32_t large;        // largeSubtreeSize
 };
 int launch_obj_bounds(const DevUpdate& u, void* stream);
-// Leaf pass, then one launch per depth (deepest first).
-int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
-                       const int32_t* levels, const int64_t* level_off, int n_levels,
+// u.leaf_at from a leaf list (the caller sets every entry to -1 first).
+int launch_leaf_starts(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
                        void* stream);
+// Leaf pass over the object slots (u.leaf_at), leaf splits, then one launch per depth
+// (deepest first).
+int launch_update_tree(const DevUpdate& u, int64_t n_leaves, const int32_t* levels,
+                       const int64_t* level_off, int n_levels, void* stream);
 // PairNodes and big leaf counts of every record pair in use, from the reference records.
 int launch_rebuild_pairs(const DevUpdate& u, void* stream);
 // Depth of every record in use by `steps` pointer-jumping steps (even, at most 8; 8 covers

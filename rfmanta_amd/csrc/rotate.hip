@@ -11,8 +11,9 @@
 // thread per node:
 //   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030), for the device
 //                    build; the update forms them from the vertices where it needs them
-//   k_upd_leaves     leaf boxes (DynBVH.cc:2091-2097), and a list of the leaves the
-//                    parent's rotation may split
+//   k_upd_leaves     one lane per object slot, the first slot of each leaf doing the leaf:
+//                    triangle records, leaf boxes (DynBVH.cc:2091-2097), and a list of
+//                    the leaves the parent's rotation may split
 //   k_split_leaves   those splits, ahead of the rotations (same subtrees)
 //   k_upd_level      one launch per depth: union of the child boxes, then rotateNode
 //   k_upd_top        the same for the small depths at the top, in one workgroup
@@ -32,6 +33,7 @@
 
 #include "approx_sah.h"
 #include "mh_device.h"
+#include "prim_setup.h"
 
 namespace mh {
 namespace {
@@ -249,27 +251,41 @@ __global__ __launch_bounds__(kBlock) void k_obj_bounds(DevUpdate u) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, const int32_t* leaves,
-                                                       int64_t n) {
+// The leaf that starts at each object slot, from a leaf list.
+__global__ __launch_bounds__(kBlock) void k_leaf_starts(DevUpdate u, const int32_t* leaves,
+                                                        int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int rec = leaves[i];
+  u.leaf_at[u.refs[rec].child] = rec;
+}
+
+// One lane per object slot; the lane at a leaf's first slot does the leaf, so neighbouring
+// lanes take neighbouring leaves and their objects' vertices and records. For each object:
+// its triangle record for the slot (what k_prim_setup writes for a build) and its box
+// (Mesh::getBBox), from one read of its vertices.
+__global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, int64_t n_slots) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_slots) return;
+  const int rec = u.leaf_at[i];
+  if (rec < 0) return;  // not the first slot of a leaf
   RefNode& nd = u.refs[rec];
   const int first = nd.child, cnt = nd.children;
   Box b;
   reset(b);
+  auto object = [&](int slot) { return obj_box(u, u.ids[slot]); };
   // The parent's rotation tries to split this leaf with the approximate SAH. Two objects:
   // decide it now, from the same object boxes (partition2Objs); more: leave the binned
   // decision to the split pass, whose registers it needs. A root leaf has no parent, a
   // one-object leaf never splits.
   bool maybe;
   if (cnt == 2) {  // the common leaf: both boxes once, for the leaf box and the decision
-    const Box o0 = obj_box(u, u.ids[first]), o1 = obj_box(u, u.ids[first + 1]);
+    const Box o0 = object(first), o1 = object(first + 1);
     extend(b, o0);
     extend(b, o1);
     maybe = pair_splits(b, o0, o1);
   } else {
-    for (int k = 0; k < cnt; ++k) extend(b, obj_box(u, u.ids[first + k]));
+    for (int k = 0; k < cnt; ++k) extend(b, object(first + k));
     maybe = cnt > 2;
   }
   set_box(nd, b);
@@ -295,8 +311,13 @@ __global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, const int32_
 // split code's registers stay out of the level kernels.
 __global__ __launch_bounds__(kBlock) void k_split_leaves(DevUpdate u) {
   const unsigned n = *u.split_count;
-  for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-    split_leaf(u, u.split_list[i]);
+  for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int rec = u.split_list[i];
+    const int b0 = u.refs[rec].child, e0 = b0 + u.refs[rec].children;
+    (void)b0;
+    (void)e0;
+    split_leaf(u, rec);
+  }
 }
 
 // One inner node of the update: the union of its child boxes, then rotateNode.
@@ -460,12 +481,14 @@ __device__ __forceinline__ void frontier_depth(const DevUpdate& u, unsigned* fr,
         put(levels, pi++, c);
       } else {
         put(leaves, pl++, c);
+        if ((unsigned)ax0.x < cap) u.leaf_at[ax0.x] = c;  // the leaf starting at its first slot
       }
       if (in1) {
         if (pi < cap) aux[pi] = ax1;
         put(levels, pi, c + 1);
       } else {
         put(leaves, pl, c + 1);
+        if ((unsigned)ax1.x < cap) u.leaf_at[ax1.x] = c + 1;
       }
     }
     __syncthreads();  // w_in, w_lf, b_in, b_lf are rewritten by the next iteration
@@ -716,13 +739,20 @@ int launch_obj_bounds(const DevUpdate& u, void* stream) {
   return (int)hipGetLastError();
 }
 
-int launch_update_tree(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
-                       const int32_t* levels, const int64_t* level_off, int n_levels,
+int launch_leaf_starts(const DevUpdate& u, const int32_t* leaves, int64_t n_leaves,
                        void* stream) {
+  if (n_leaves > 0)
+    hipLaunchKernelGGL(k_leaf_starts, dim3(grid(n_leaves)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), u, leaves, n_leaves);
+  return (int)hipGetLastError();
+}
+
+int launch_update_tree(const DevUpdate& u, int64_t n_leaves, const int32_t* levels,
+                       const int64_t* level_off, int n_levels, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (n_leaves > 0) {
-    hipLaunchKernelGGL(k_upd_leaves, dim3(grid(n_leaves)), dim3(kBlock), 0, s, u, leaves,
-                       n_leaves);
+  if (n_leaves > 0 && u.n_objects > 0) {
+    hipLaunchKernelGGL(k_upd_leaves, dim3(grid(u.n_objects)), dim3(kBlock), 0, s, u,
+                       (int64_t)u.n_objects);
     const unsigned g = grid(n_leaves);  // the list holds at most n_leaves entries
     hipLaunchKernelGGL(k_split_leaves, dim3(g < 1024 ? g : 1024), dim3(kBlock), 0, s, u);
   }

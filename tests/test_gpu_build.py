@@ -86,6 +86,47 @@ def test_device_build_then_update(oracle):
             assert ok, (f, msg)
 
 
+def test_rebuild_between_updates(oracle):
+    """Updates after a host build, then a device rebuild of the displaced mesh, then
+    updates again: every tree equals the oracle's (the update's per-slot leaf map and level
+    lists are set up again for the new tree)."""
+    from test_gpu_update import canonical_equal
+    v, i = scenes.soup(3000, seed=21)
+
+    def frame(f):
+        vd = v.copy()
+        vd[:, 1] += (0.3 * np.sin(4 * v[:, 0] + 0.7 * f)).astype(np.float32)
+        return vd
+
+    with api.Scene(v, i) as s:
+        s.build()
+        ids, nodes = s.export_dynbvh()
+        st = oracle.RotState(nodes, ids, s.stats()["nodes"])
+        for f in range(2):
+            vd = frame(f)
+            st.update(vd, i)
+            s.set_vertices(vd)
+            s.update()
+            api.sync()
+            gi, gn = s.export_dynbvh()
+            ok, msg, _ = canonical_equal(gn, gi, st.nodes, st.ids)
+            assert ok, ("host build", f, msg)
+        vd = frame(2)
+        s.set_vertices(vd)
+        s.build(approximate=True)
+        on, oi, onn = oracle.build_bvh_approx(vd, i)
+        st = oracle.RotState(on, oi, onn)
+        for f in range(3, 5):
+            vd = frame(f)
+            st.update(vd, i)
+            s.set_vertices(vd)
+            s.update()
+            api.sync()
+            gi, gn = s.export_dynbvh()
+            ok, msg, _ = canonical_equal(gn, gi, st.nodes, st.ids)
+            assert ok, ("device rebuild", f, msg)
+
+
 def test_device_build_1m_invariants():
     """997,002-triangle heightfield: boxes are exact unions, every object in one sorted
     leaf, every record reachable, and the closest hits of the full-SAH tree and the
