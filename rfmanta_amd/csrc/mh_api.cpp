@@ -947,8 +947,6 @@ int mh_scene_update(mh_scene* s, void* stream) {
   u.split_list = s->split_list.as<int32_t>();
   u.split_count = nfree + 2;
   u.leaf_at = s->leaf_at.as<int32_t>();
-  u.tris = s->tris.as<float>();
-  u.prim_kind = s->prim_kind;
   u.n_records = (int32_t)nrec;
   u.n_objects = (int32_t)T;
   // largeSubtreeSize = nodes.size() / (20 * numProcs) (DynBVH.cc:1776)

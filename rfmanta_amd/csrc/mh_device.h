@@ -181,8 +181,6 @@ struct DevUpdate {
   int32_t* split_list;   // update: leaves whose parent's rotation may split them
   unsigned* split_count; // update: entries of split_list (cleared by the host)
   int32_t* leaf_at;      // update: per slot, the leaf whose first slot it is, else -1
-  float* tris;           // update: triangle records, rewritten from the current vertices
-  int32_t prim_kind;     // update: MH_PRIM_* of the records
   int32_t n_records;     // capacity, 2T
   int32_t n_objects;     // T
   uint32_t large;        // largeSubtreeSize

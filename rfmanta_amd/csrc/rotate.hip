@@ -12,8 +12,8 @@
 //   k_obj_bounds     per object: box and centroid (DynBVH.cc:2018-2030), for the device
 //                    build; the update forms them from the vertices where it needs them
 //   k_upd_leaves     one lane per object slot, the first slot of each leaf doing the leaf:
-//                    triangle records, leaf boxes (DynBVH.cc:2091-2097), and a list of
-//                    the leaves the parent's rotation may split
+//                    leaf boxes (DynBVH.cc:2091-2097), and a list of the leaves the
+//                    parent's rotation may split
 //   k_split_leaves   those splits, ahead of the rotations (same subtrees)
 //   k_upd_level      one launch per depth: union of the child boxes, then rotateNode
 //   k_upd_top        the same for the small depths at the top, in one workgroup
@@ -33,7 +33,6 @@
 
 #include "approx_sah.h"
 #include "mh_device.h"
-#include "prim_setup.h"
 
 namespace mh {
 namespace {
@@ -261,9 +260,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_starts(DevUpdate u, const int32
 }
 
 // One lane per object slot; the lane at a leaf's first slot does the leaf, so neighbouring
-// lanes take neighbouring leaves and their objects' vertices and records. For each object:
-// its triangle record for the slot (what k_prim_setup writes for a build) and its box
-// (Mesh::getBBox), from one read of its vertices.
+// lanes take neighbouring leaves and gather neighbouring vertices. Object boxes come from
+// the vertices (Mesh::getBBox).
 __global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, int64_t n_slots) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n_slots) return;
@@ -311,13 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_upd_leaves(DevUpdate u, int64_t n_sl
 // split code's registers stay out of the level kernels.
 __global__ __launch_bounds__(kBlock) void k_split_leaves(DevUpdate u) {
   const unsigned n = *u.split_count;
-  for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const int rec = u.split_list[i];
-    const int b0 = u.refs[rec].child, e0 = b0 + u.refs[rec].children;
-    (void)b0;
-    (void)e0;
-    split_leaf(u, rec);
-  }
+  for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    split_leaf(u, u.split_list[i]);
 }
 
 // One inner node of the update: the union of its child boxes, then rotateNode.
