@@ -75,3 +75,11 @@ def test_update_built_tree_host(update_host, oracle, tmp_path):
     nodes, ids, nn = oracle.build_bvh(v, i)
     vd = (v + np.random.default_rng(3).normal(0, 0.3, v.shape)).astype(np.float32)
     _run(update_host, tmp_path / "b", oracle, nodes, ids, nn, vd, i)
+
+
+def test_update_heightfield_host(update_host, oracle, tmp_path):
+    v, i = scenes.heightfield(120, 60, seed=4)
+    nodes, ids, nn = oracle.build_bvh(v, i)
+    vd = v.copy()
+    vd[:, 1] += (0.05 * np.sin(4 * v[:, 0] + 0.3)).astype(np.float32)
+    _run(update_host, tmp_path / "c", oracle, nodes, ids, nn, vd, i)
